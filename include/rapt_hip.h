@@ -1,0 +1,160 @@
+/*
+ * rapt_hip.h -- C-ABI of librapt_hip.so, the MI355X (gfx950) particle tracer.
+ *
+ * The reference (mkozturk/rapt) is pure Python; its only native boundary on
+ * the tracing path is scipy.integrate.ode driving a Python right-hand side
+ * (rapt/Particle.py:300-305, rapt/GuidingCenter.py:450-453).  This library
+ * replaces that boundary for a whole batch of particles at once: the Python
+ * front end (rapt_amd/) packs the tracer objects' last trajectory rows and
+ * rapt.params into the structs below and calls these entry points via ctypes.
+ *
+ * Conventions: plain pointers and sizes, no C++ types; every function returns
+ * 0 on success and a negative RAPT_E_* code on failure (message via
+ * rapt_last_error).  Host pointers are caller-owned; device buffers are owned
+ * by the library (rapt_batch).  A context is bound to one device; calls on one
+ * context are not thread-safe.
+ */
+#ifndef RAPT_HIP_H
+#define RAPT_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define RAPT_OK 0
+#define RAPT_E_ARG (-1)
+#define RAPT_E_HIP (-2)
+#define RAPT_E_OOM (-3)
+#define RAPT_E_STATE (-4)
+
+/* ---- field models (rapt/fields.py:282-511 plus two user-analytic forms) */
+enum rapt_field_model {
+    RAPT_FIELD_ZERO = 0,
+    RAPT_FIELD_EARTH_DIPOLE = 1,       /* p0 = -3 B0 Re^3          rapt/fields.py:282-317 */
+    RAPT_FIELD_DOUBLE_DIPOLE = 2,      /* p0 = -B0 Re^3, p1 = distance, p2 = imagestrength
+                                                                   rapt/fields.py:319-362 */
+    RAPT_FIELD_UNIFORM_BZ = 3,         /* p0 = Bz                  rapt/fields.py:364-390 */
+    RAPT_FIELD_UNIFORM_CROSSED_EB = 4, /* p0 = Bz, p1 = Ey         rapt/fields.py:392-427 */
+    RAPT_FIELD_VAR_EARTH_DIPOLE = 5,   /* p0 = -B0 Re^3, p1 = amp, p2 = period
+                                                                   rapt/fields.py:429-470 */
+    RAPT_FIELD_PARABOLIC = 6,          /* p0 = B0, p1 = Bn, p2 = d, p3 = module B0
+                                                                   rapt/fields.py:472-511 */
+    RAPT_FIELD_CURRENT_SHEET = 7,      /* B = (p0 tanh(z/p1), 0, p2), E = (p3, p4, p5) */
+    RAPT_FIELD_DIPOLE_UNIFORM = 8,     /* EarthDipole(p0) + (p1, p2, p3)                */
+    RAPT_FIELD_NMODELS = 9
+};
+
+/* Field descriptor: replaces the _Field plugin object (rapt/fields.py:5-41). */
+typedef struct {
+    int32_t model;      /* enum rapt_field_model */
+    int32_t is_static;  /* _Field.static */
+    double gstep;       /* _Field.gradientstepsize */
+    double tstep;       /* _Field.timederivstepsize */
+    double p[16];       /* model constants */
+} rapt_field_desc;
+
+/* ---- tracing modes ---------------------------------------------------- */
+enum rapt_mode {
+    RAPT_MODE_PARTICLE = 0, /* Particle.advance     rapt/Particle.py:230-309 */
+    RAPT_MODE_GC = 1,       /* GuidingCenter.advance rapt/GuidingCenter.py:397-458 */
+    RAPT_MODE_ADAPTIVE = 2  /* Adaptive.advance     rapt/Adaptive.py:187-222 */
+};
+enum rapt_eom { RAPT_EOM_TCB = 0, RAPT_EOM_BC = 1, RAPT_EOM_NT = 2 };
+enum rapt_stepper { RAPT_STEP_DOP853 = 0, RAPT_STEP_DOPRI5 = 1, RAPT_STEP_RKF45 = 2 };
+enum rapt_precision { RAPT_PREC_FP64 = 0, RAPT_PREC_FP32_FIELD = 1 };
+
+/* Per-launch settings: rapt.params (rapt/__init__.py:21-34) plus the scipy
+ * set_integrator arguments the tracers pass (rapt/Particle.py:300,
+ * rapt/GuidingCenter.py:450). */
+typedef struct {
+    int32_t mode;                /* enum rapt_mode */
+    int32_t eom;                 /* enum rapt_eom (GC segments) */
+    int32_t stepper;             /* stepper of the mode's primary segment */
+    int32_t precision;           /* enum rapt_precision */
+    int32_t nsteps;              /* scipy nsteps (500) */
+    int32_t enforce_equatorial;  /* params["enforce equatorial"] */
+    int32_t check_adiabaticity;  /* raise Adiabatic / NonAdiabatic -> stop (PARTICLE/GC) */
+    int32_t block;               /* threads per block (0 = default 256) */
+    double rtol, atol;           /* params["solvertolerances"] */
+    double safety, dfactor, ifactor, beta;  /* scipy dop853/dopri5 knobs */
+    double max_step, first_step;
+    double gc_rtol, gc_atol;     /* ADAPTIVE: tolerances of GC segments */
+    double rkf_tol, rkf_hmax, rkf_hmin;     /* RKF45 stepper (rapt/rkf.py) */
+    double cyclotronresolution;  /* params["cyclotronresolution"] */
+    double gctimestep;           /* params["GCtimestep"] (ADAPTIVE GC segments) */
+    double epss, epst;           /* params["epss"], params["epst"] */
+    double delta;                /* advance(delta) */
+    int64_t lanes;               /* resident lanes of the persistent grid (0 = auto);
+                                    fewer lanes than particles turns on work refill */
+} rapt_trace_desc;
+
+/* Per-particle state, slot-major (state[slot * n + i]).  Filled by the host
+ * from the tracer's last trajectory row; updated in place by rapt_batch_run so
+ * a run stopped at its row capacity continues where it stopped. */
+enum rapt_slot {
+    RAPT_S_T = 0,                   /* r.t (integrator time) */
+    RAPT_S_Y0 = 1,                  /* x, y, z, px, py, pz | x, y, z, ppar */
+    RAPT_S_MASS = 7, RAPT_S_CHARGE = 8,
+    RAPT_S_MU = 9,                  /* GC: magnetic moment */
+    RAPT_S_V = 10,                  /* GC: speed (BrizardChan / NorthropTeller) */
+    RAPT_S_DT = 11,                 /* output step (GC input; Particle: computed) */
+    RAPT_S_H = 12,                  /* step size carried between integrate() calls */
+    RAPT_S_TEND = 13,               /* t0 + delta of the current advance */
+    RAPT_S_GM = 14,                 /* gamma*m frozen at advance start (static fields) */
+    RAPT_S_TCUR = 15,               /* tracer.tcur */
+    RAPT_NSLOT = 16
+};
+/* per-particle status */
+enum rapt_status {
+    RAPT_ST_PENDING = 0, RAPT_ST_DONE = 1, RAPT_ST_SWITCH = 2,
+    RAPT_ST_NSTEPS = -2, RAPT_ST_HSMALL = -3, RAPT_ST_NAN = -4, RAPT_ST_GCITER = -5,
+    RAPT_ST_CAP = -6, RAPT_ST_HMIN = -7, RAPT_ST_CONFIG = -10
+};
+
+typedef struct {
+    int64_t naccpt, nrejct, nstep, nfcn;  /* summed over the batch */
+    int64_t nswitch, nrows;
+    double kernel_ms;                     /* device time of the launch (HIP events) */
+} rapt_counters;
+
+typedef struct rapt_ctx rapt_ctx;
+typedef struct rapt_batch rapt_batch;
+
+/* ---- context ---------------------------------------------------------- */
+int rapt_ctx_create(int device, rapt_ctx **out);
+int rapt_ctx_destroy(rapt_ctx *ctx);
+const char *rapt_last_error(rapt_ctx *ctx);
+/* replaces the field object handed to the tracer constructors */
+int rapt_field_set(rapt_ctx *ctx, const rapt_field_desc *field);
+
+/* Batched field evaluation on the device (the _Field methods,
+ * rapt/fields.py:76-280).  what: 0 B, 1 E, 2 unitb, 3 magB, 4 gradB,
+ * 5 jacobianB, 6 curvature, 7 curlb, 8 dBdt, 9 dbdt, 10 lengthscale,
+ * 11 timescale.  tpos: n x 4 host array; out: n x width(what) host array. */
+int rapt_field_eval(rapt_ctx *ctx, int what, int64_t n, const double *tpos, double *out);
+
+/* ---- device-resident batch (replaces one scipy ode loop per particle) -- */
+int rapt_batch_create(rapt_ctx *ctx, int64_t n, rapt_batch **out);
+int rapt_batch_destroy(rapt_batch *b);
+/* state: RAPT_NSLOT x n slot-major host array; flags: per-particle int32
+ * (bit 0: advance already started; bits 1-2: adaptive segment mode). */
+int rapt_batch_upload(rapt_batch *b, const double *state, const int32_t *flags);
+/* Row storage: row_cap[i] rows of `width` doubles for particle i (width 7
+ * PARTICLE, 5 GC, 8 ADAPTIVE).  Resets the per-particle row counters. */
+int rapt_batch_set_rows(rapt_batch *b, int32_t width, const int64_t *row_cap);
+/* Run the advance on the device; inputs stay in HBM.  counters may be NULL. */
+int rapt_batch_run(rapt_batch *b, const rapt_trace_desc *desc, rapt_counters *counters);
+/* Copy results to the host (any pointer may be NULL). */
+int rapt_batch_download(rapt_batch *b, double *state, int32_t *flags, int32_t *status,
+                        int64_t *nrows, double *rows, int64_t *naccpt);
+
+/* One-shot convenience: upload, run, download (host buffers). */
+int rapt_trace(rapt_ctx *ctx, const rapt_trace_desc *desc, int64_t n, double *state,
+               int32_t *flags, const int64_t *row_cap, double *rows, int64_t *nrows,
+               int32_t *status, rapt_counters *counters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

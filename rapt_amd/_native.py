@@ -1,0 +1,258 @@
+"""ctypes binding of librapt_hip.so (include/rapt_hip.h).
+
+The library is required: importing the tracers works without it, but the
+first call that needs the GPU raises RuntimeError if the library is missing or
+no HIP device is visible.  There is no CPU integration path.
+"""
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "librapt_hip.so")
+
+MODE_PARTICLE, MODE_GC, MODE_ADAPTIVE = 0, 1, 2
+EOM = {"TaoChanBrizardEOM": 0, "BrizardChanEOM": 1, "NorthropTellerEOM": 2}
+STEP_DOP853, STEP_DOPRI5, STEP_RKF45 = 0, 1, 2
+NSLOT = 16
+S_T, S_Y0, S_MASS, S_CHARGE, S_MU, S_V, S_DT, S_H, S_TEND, S_GM, S_TCUR = \
+    0, 1, 7, 8, 9, 10, 11, 12, 13, 14, 15
+FLAG_STARTED, FLAG_GCMODE, FLAG_AINIT = 1, 2, 4
+ST_PENDING, ST_DONE, ST_SWITCH = 0, 1, 2
+ST_NSTEPS, ST_HSMALL, ST_NAN, ST_GCITER, ST_CAP, ST_HMIN = -2, -3, -4, -5, -6, -7
+STATUS_TEXT = {1: "ok", 2: "adiabaticity switch", -2: "larger nsteps is needed",
+               -3: "step size becomes too small", -4: "non-finite state",
+               -5: "guiding-centre iteration did not converge", -6: "row capacity reached",
+               -7: "step below hmin", -10: "bad configuration"}
+EVAL = dict(B=0, E=1, unitb=2, magB=3, gradB=4, jacobianB=5, curvature=6, curlb=7,
+            dBdt=8, dbdt=9, lengthscale=10, timescale=11)
+EVAL_WIDTH = dict(B=3, E=3, unitb=3, magB=1, gradB=3, jacobianB=9, curvature=1, curlb=3,
+                  dBdt=1, dbdt=3, lengthscale=1, timescale=1)
+ROW_WIDTH = {MODE_PARTICLE: 7, MODE_GC: 5, MODE_ADAPTIVE: 8}
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [("model", C.c_int32), ("is_static", C.c_int32), ("gstep", C.c_double),
+                ("tstep", C.c_double), ("p", C.c_double * 16)]
+
+
+class TraceDesc(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("eom", C.c_int32), ("stepper", C.c_int32),
+                ("precision", C.c_int32), ("nsteps", C.c_int32),
+                ("enforce_equatorial", C.c_int32), ("check_adiabaticity", C.c_int32),
+                ("block", C.c_int32), ("rtol", C.c_double), ("atol", C.c_double),
+                ("safety", C.c_double), ("dfactor", C.c_double), ("ifactor", C.c_double),
+                ("beta", C.c_double), ("max_step", C.c_double), ("first_step", C.c_double),
+                ("gc_rtol", C.c_double), ("gc_atol", C.c_double), ("rkf_tol", C.c_double),
+                ("rkf_hmax", C.c_double), ("rkf_hmin", C.c_double),
+                ("cyclotronresolution", C.c_double), ("gctimestep", C.c_double),
+                ("epss", C.c_double), ("epst", C.c_double), ("delta", C.c_double),
+                ("lanes", C.c_int64)]
+
+
+class Counters(C.Structure):
+    _fields_ = [("naccpt", C.c_int64), ("nrejct", C.c_int64), ("nstep", C.c_int64),
+                ("nfcn", C.c_int64), ("nswitch", C.c_int64), ("nrows", C.c_int64),
+                ("kernel_ms", C.c_double)]
+
+
+_lib = None
+_lock = threading.Lock()
+_ctx = {}
+
+
+def lib():
+    """Load librapt_hip.so (raises RuntimeError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("librapt_hip.so is not built (%s); run __graft_entry__.build() or "
+                           "make -C rapt_amd/csrc" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp, dp, i32p, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), \
+        C.POINTER(C.c_int64)
+    L.rapt_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.rapt_ctx_destroy.argtypes = [vp]
+    L.rapt_last_error.argtypes = [vp]
+    L.rapt_last_error.restype = C.c_char_p
+    L.rapt_field_set.argtypes = [vp, C.POINTER(FieldDesc)]
+    L.rapt_field_eval.argtypes = [vp, C.c_int, C.c_int64, dp, dp]
+    L.rapt_batch_create.argtypes = [vp, C.c_int64, C.POINTER(vp)]
+    L.rapt_batch_destroy.argtypes = [vp]
+    L.rapt_batch_upload.argtypes = [vp, dp, i32p]
+    L.rapt_batch_set_rows.argtypes = [vp, C.c_int32, i64p]
+    L.rapt_batch_run.argtypes = [vp, C.POINTER(TraceDesc), C.POINTER(Counters)]
+    L.rapt_batch_download.argtypes = [vp, dp, i32p, i32p, i64p, dp, i64p]
+    L.rapt_trace.argtypes = [vp, C.POINTER(TraceDesc), C.c_int64, dp, i32p, i64p, dp, i64p,
+                             i32p, C.POINTER(Counters)]
+    for nm in ("rapt_ctx_create", "rapt_ctx_destroy", "rapt_field_set", "rapt_field_eval",
+               "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
+               "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace"):
+        getattr(L, nm).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _ip64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+class Context:
+    """One library context (HIP stream + events) on one device."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.rapt_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise RuntimeError("rapt_ctx_create(device=%d) failed with %d: no usable HIP "
+                               "device" % (device, rc))
+        self.h = h
+        self.device = device
+        self._field_key = None
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = lib().rapt_last_error(self.h)
+            raise RuntimeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def set_field(self, desc):
+        key = (desc["model"], desc["static"], desc["gstep"], desc["tstep"], desc["params"])
+        if key == self._field_key:
+            return
+        f = FieldDesc()
+        f.model = desc["model"]
+        f.is_static = 1 if desc["static"] else 0
+        f.gstep = desc["gstep"]
+        f.tstep = desc["tstep"]
+        for i, v in enumerate(desc["params"]):
+            f.p[i] = v
+        self.check(lib().rapt_field_set(self.h, C.byref(f)), "rapt_field_set")
+        self._field_key = key
+
+    def field_eval(self, desc, what, tpos):
+        self.set_field(desc)
+        tpos = np.ascontiguousarray(np.atleast_2d(tpos), dtype=np.float64)
+        n = tpos.shape[0]
+        w = EVAL_WIDTH[what]
+        out = np.zeros((n, w))
+        self.check(lib().rapt_field_eval(self.h, EVAL[what], n, _dp(tpos), _dp(out)),
+                   "rapt_field_eval")
+        return out
+
+
+def context(device=0):
+    with _lock:
+        if device not in _ctx:
+            _ctx[device] = Context(device)
+        return _ctx[device]
+
+
+class Batch:
+    """Device-resident particle batch (rapt_batch_*)."""
+
+    def __init__(self, ctx, n):
+        self.ctx = ctx
+        self.n = int(n)
+        h = C.c_void_p()
+        ctx.check(lib().rapt_batch_create(ctx.h, self.n, C.byref(h)), "rapt_batch_create")
+        self.h = h
+        self.width = None
+
+    def close(self):
+        if self.h:
+            lib().rapt_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, state, flags):
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        flags = np.ascontiguousarray(flags, dtype=np.int32)
+        assert state.shape == (NSLOT, self.n) and flags.shape == (self.n,)
+        self.ctx.check(lib().rapt_batch_upload(self.h, _dp(state), _ip32(flags)),
+                       "rapt_batch_upload")
+
+    def set_rows(self, width, row_cap):
+        row_cap = np.ascontiguousarray(row_cap, dtype=np.int64)
+        assert row_cap.shape == (self.n,)
+        self.ctx.check(lib().rapt_batch_set_rows(self.h, width, _ip64(row_cap)),
+                       "rapt_batch_set_rows")
+        self.width = width
+        self.row_cap = row_cap
+        self.row_off = np.concatenate([[0], np.cumsum(row_cap)[:-1]]).astype(np.int64)
+
+    def run(self, desc):
+        cnt = Counters()
+        self.ctx.check(lib().rapt_batch_run(self.h, C.byref(desc), C.byref(cnt)),
+                       "rapt_batch_run")
+        return cnt
+
+    def download(self, rows=True):
+        n = self.n
+        state = np.zeros((NSLOT, n))
+        flags = np.zeros(n, np.int32)
+        status = np.zeros(n, np.int32)
+        nrows = np.zeros(n, np.int64)
+        naccpt = np.zeros(n, np.int64)
+        rbuf = np.zeros((int(self.row_cap.sum()), self.width)) if rows else None
+        self.ctx.check(lib().rapt_batch_download(
+            self.h, _dp(state), _ip32(flags), _ip32(status), _ip64(nrows),
+            _dp(rbuf) if rows and rbuf.size else None, _ip64(naccpt)), "rapt_batch_download")
+        return dict(state=state, flags=flags, status=status, nrows=nrows, naccpt=naccpt,
+                    rows=rbuf)
+
+
+def make_desc(mode, params, delta, eom="TaoChanBrizardEOM", stepper=None, check=False,
+              lanes=0, block=0, **kw):
+    """TraceDesc from a rapt params dict (rapt/__init__.py:21-34) and the scipy
+    integrator settings the tracers use (rapt/Particle.py:300,
+    rapt/GuidingCenter.py:450)."""
+    d = TraceDesc()
+    d.mode = mode
+    d.eom = EOM[eom] if isinstance(eom, str) else int(eom)
+    if stepper is None:
+        stepper = STEP_DOPRI5 if mode == MODE_GC else STEP_DOP853
+    d.stepper = stepper
+    d.precision = 0
+    d.nsteps = int(kw.get("nsteps", 500))
+    d.enforce_equatorial = 1 if params.get("enforce equatorial", False) else 0
+    d.check_adiabaticity = 1 if check else 0
+    d.block = block
+    rtol, atol = params["solvertolerances"]
+    d.rtol, d.atol = float(rtol), float(atol)
+    d.gc_rtol, d.gc_atol = float(rtol), float(atol)
+    if mode == MODE_GC:
+        d.safety, d.dfactor, d.ifactor, d.beta = 0.9, 0.2, 10.0, 0.0
+    else:
+        d.safety, d.dfactor, d.ifactor, d.beta = 0.9, 0.3, 6.0, 0.1
+    for k in ("safety", "dfactor", "ifactor", "beta"):
+        if k in kw:
+            setattr(d, k, float(kw[k]))
+    d.max_step = float(kw.get("max_step", 0.0))
+    d.first_step = float(kw.get("first_step", 0.0))
+    d.rkf_tol = float(kw.get("rkf_tol", 1e-3))
+    d.rkf_hmax = float(kw.get("rkf_hmax", 0.0))
+    d.rkf_hmin = float(kw.get("rkf_hmin", 1e-6))
+    d.cyclotronresolution = float(params["cyclotronresolution"])
+    d.gctimestep = float(params.get("GCtimestep", 0.0))
+    d.epss = float(params["epss"])
+    d.epst = float(params["epst"])
+    d.delta = float(delta)
+    d.lanes = int(lanes)
+    return d

@@ -1,0 +1,301 @@
+"""Write golden fixtures by running the reference (mkozturk/rapt) in this container.
+
+Run from the repository root:  python tests/golden/make_golden.py
+
+The reference is imported from /root/reference as a library; its module
+rapt/flutils.py imports ``scipy.misc.derivative``, which scipy 1.15 no longer
+has, so a stand-in that raises is installed first (derivative() is only used
+by flutils.criticalpoints, which no fixture calls).  Inputs are given as Python
+lists (rapt/Particle.py:105 compares ``pos == []``).  The fixtures are data
+only: inputs and the reference's outputs, written to tests/golden/*.npz.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+import scipy.misc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+
+def _no_derivative(*a, **k):
+    raise NotImplementedError("scipy.misc.derivative is not available")
+
+
+if not hasattr(scipy.misc, "derivative"):
+    scipy.misc.derivative = _no_derivative
+sys.path.insert(0, "/root/reference")
+
+import rapt  # noqa: E402  (the reference)
+from rapt import fields as rf, utils as ru, flutils as rfu  # noqa: E402
+from rapt.rkf import rkf as ref_rkf  # noqa: E402
+
+from rapt_amd import workloads as W  # noqa: E402  (seeded inputs only)
+
+Re = rapt.Re
+DEFAULT_PARAMS = dict(rapt.params)
+
+
+def reset_params(**kw):
+    rapt.params.clear()
+    rapt.params.update(DEFAULT_PARAMS)
+    rapt.params.update(kw)
+
+
+class RefCurrentSheet(rf._Field):
+    """B = (B0 tanh(z/lam), 0, Bn), E uniform: a user field written the
+    reference's way (override B and E, set static and gradientstepsize)."""
+
+    def __init__(self, B0=20e-9, lam=0.5 * Re, Bn=1e-9, E=(0.0, 0.3e-3, 0.0)):
+        rf._Field.__init__(self)
+        self.B0, self.lam, self.Bn = B0, lam, Bn
+        self._E = np.array(E, dtype=float)
+        self.static = not np.any(self._E != 0)
+        self.gradientstepsize = lam / 1000
+
+    def B(self, tpos):
+        z = tpos[3]
+        return np.array([self.B0 * np.tanh(z / self.lam), 0, self.Bn])
+
+    def E(self, tpos):
+        return self._E.copy()
+
+
+class RefDipolePlusUniform(rf.EarthDipole):
+    def __init__(self, Bu=(0.0, 0.0, -10e-9)):
+        rf.EarthDipole.__init__(self)
+        self._Bu = np.array(Bu, dtype=float)
+
+    def B(self, tpos):
+        return rf.EarthDipole.B(self, tpos) + self._Bu
+
+
+def field_cases():
+    return {
+        "EarthDipole": (rf.EarthDipole(), "dipole"),
+        "DoubleDipole": (rf.DoubleDipole(), "dipole"),
+        "DoubleDipole_k2": (rf.DoubleDipole(distance=15 * Re, imagestrength=2), "dipole"),
+        "UniformBz": (rf.UniformBz(Bz=2e-7), "dipole"),
+        "UniformCrossedEB": (rf.UniformCrossedEB(Ey=1e-3, Bz=1e-7), "dipole"),
+        "VarEarthDipole": (rf.VarEarthDipole(amp=0.2, period=7.0), "dipole"),
+        "Parabolic": (rf.Parabolic(B0=10.0, Bn=1.0, d=0.2), "parabolic"),
+        "CurrentSheet": (RefCurrentSheet(), "sheet"),
+        "DipolePlusUniform": (RefDipolePlusUniform(), "dipole"),
+    }
+
+
+def sample_points(kind, n, seed):
+    u = lambda s, lo, hi: W.uniform(seed, s, n, lo, hi)
+    t = u(0, 0.0, 20.0)
+    if kind == "dipole":
+        r = u(1, 2.0, 9.0) * Re
+        th = np.arccos(u(2, -0.8, 0.8))
+        ph = u(3, 0.0, 2 * np.pi)
+        pts = np.stack([t, r * np.sin(th) * np.cos(ph), r * np.sin(th) * np.sin(ph),
+                        r * np.cos(th)], axis=1)
+    elif kind == "parabolic":
+        pts = np.stack([t, u(1, -3, 3), u(2, -3, 3), u(3, -2.0, 2.0)], axis=1)
+    else:
+        pts = np.stack([t, u(1, -5, 5) * Re, u(2, -5, 5) * Re, u(3, -1.5, 1.5) * Re], axis=1)
+    return pts
+
+
+def make_fields():
+    out = {}
+    what = ["B", "E", "unitb", "magB", "gradB", "jacobianB", "curvature", "curlb", "dBdt",
+            "dbdt", "lengthscale", "timescale"]
+    for name, (f, kind) in field_cases().items():
+        pts = sample_points(kind, 24, 11)
+        out[name + "/tpos"] = pts
+        for w in what:
+            vals = []
+            for tp in pts:
+                v = getattr(f, w)(np.array(tp, dtype=float))
+                if v is None:
+                    v = np.nan
+                vals.append(np.array(v, dtype=float).reshape(-1))
+            out[name + "/" + w] = np.array(vals)
+    np.savez_compressed(os.path.join(HERE, "fields.npz"), **out)
+
+
+def run_particle(f, pos, vel, mass, charge, delta, **params):
+    reset_params(**params)
+    p = rapt.Particle(list(map(float, pos)), list(map(float, vel)), 0, mass, charge, f)
+    p.advance(delta)
+    return p.trajectory.copy(), p.tcur
+
+
+def make_particles():
+    out = {}
+    wl = W.dipole_protons_1mev(n=6)
+    trajs, tcurs = [], []
+    for i in range(6):
+        tr, tc = run_particle(rf.EarthDipole(), wl["pos"][i], wl["vel"][i], rapt.m_pr, rapt.e,
+                              10.0, cyclotronresolution=20)
+        trajs.append(tr)
+        tcurs.append(tc)
+    out["c1/pos"], out["c1/vel"] = wl["pos"], wl["vel"]
+    out["c1/len"] = np.array([len(t) for t in trajs])
+    out["c1/traj"] = np.concatenate(trajs)
+    out["c1/tcur"] = np.array(tcurs)
+    # other fields and settings, shorter runs
+    cases = []
+    cases.append(("crossedEB", rf.UniformCrossedEB(Ey=1e-3, Bz=1e-7), [0.0, 0.0, 0.0],
+                  [2e4, 1e4, 3e3], rapt.m_pr, rapt.e, 5e-3, {}))
+    cases.append(("vardipole", rf.VarEarthDipole(amp=0.2, period=7.0), [5 * Re, 0, 0],
+                  [0, 4e6, 9e6], rapt.m_pr, rapt.e, 1.0, {}))
+    cs = W.current_sheet_protons(n=2)
+    cases.append(("sheet", RefCurrentSheet(), cs["pos"][0], cs["vel"][0], rapt.m_pr, rapt.e,
+                  20.0, {}))
+    cases.append(("equatorial", rf.EarthDipole(), [6 * Re, 0, 0], [0, 1.2e7, 3e6], rapt.m_pr,
+                  rapt.e, 2.0, {"enforce equatorial": True}))
+    mx = W.mixed_dipole_uniform(n=4)
+    cases.append(("dipunif_e", RefDipolePlusUniform(), mx["pos"][1], mx["vel"][1], rapt.m_el,
+                  -rapt.e, 2e-4, {}))
+    cases.append(("dipunif_p", RefDipolePlusUniform(), mx["pos"][0], mx["vel"][0], rapt.m_pr,
+                  rapt.e, 2.0, {}))
+    for name, f, pos, vel, m, q, delta, prm in cases:
+        tr, tc = run_particle(f, pos, vel, m, q, delta, **prm)
+        out[name + "/pos"] = np.array(pos, float)
+        out[name + "/vel"] = np.array(vel, float)
+        out[name + "/mq"] = np.array([m, q, delta])
+        out[name + "/traj"] = tr
+        out[name + "/tcur"] = np.array([tc])
+    reset_params()
+    np.savez_compressed(os.path.join(HERE, "particles.npz"), **out)
+
+
+def make_gcs():
+    out = {}
+    wl = W.doubledipole_electrons_gc(n=6)
+    trajs, dts, mus, tcurs = [], [], [], []
+    for i in range(6):
+        reset_params(solvertolerances=(1e-8, 1e-8))
+        g = rapt.GuidingCenter(list(map(float, wl["pos"][i])), float(wl["v"][i]),
+                               float(wl["pa"][i]), None, 0, rapt.m_el, -rapt.e,
+                               rf.DoubleDipole())
+        dt = g.bounceperiod() / rapt.params["bounceresolution"]
+        g.advance(3.0)
+        trajs.append(g.trajectory.copy())
+        dts.append(dt)
+        mus.append(g.mu)
+        tcurs.append(g.tcur)
+    out["c2/pos"], out["c2/v"], out["c2/pa"] = wl["pos"], wl["v"], wl["pa"]
+    out["c2/len"] = np.array([len(t) for t in trajs])
+    out["c2/traj"] = np.concatenate(trajs)
+    out["c2/dt"] = np.array(dts)
+    out["c2/mu"] = np.array(mus)
+    out["c2/tcur"] = np.array(tcurs)
+    # eom variants, other fields, GCtimestep
+    cases = [
+        ("bc", rf.DoubleDipole(), 1, "BrizardChanEOM", 2.0, {"solvertolerances": (1e-8, 1e-8),
+                                                              "GCtimestep": 0.05}),
+        ("nt", rf.DoubleDipole(), 2, "NorthropTellerEOM", 2.0, {"solvertolerances": (1e-8, 1e-8),
+                                                                 "GCtimestep": 0.05}),
+        ("dipunif", RefDipolePlusUniform(), 3, "TaoChanBrizardEOM", 2.0, {"GCtimestep": 0.02}),
+        ("earth", rf.EarthDipole(), 4, "TaoChanBrizardEOM", 2.0, {}),
+    ]
+    for name, f, i, eom, delta, prm in cases:
+        reset_params(**prm)
+        g = rapt.GuidingCenter(list(map(float, wl["pos"][i])), float(wl["v"][i]),
+                               float(wl["pa"][i]), None, 0, rapt.m_el, -rapt.e, f)
+        dt = prm.get("GCtimestep") or g.bounceperiod() / rapt.params["bounceresolution"]
+        g.advance(delta, eom=eom)
+        out[name + "/traj"] = g.trajectory.copy()
+        out[name + "/in"] = np.array([wl["v"][i], wl["pa"][i], g.mu, dt, delta, g.tcur])
+        out[name + "/pos"] = wl["pos"][i]
+    # non-static field (E != 0): current sheet proton GC, TCB with E x b drift
+    reset_params(GCtimestep=0.5)
+    f = RefCurrentSheet()
+    g = rapt.GuidingCenter([1.0 * Re, 2.0 * Re, 0.9 * Re], 4.0e5, 40.0, None, 0, rapt.m_pr,
+                           rapt.e, f)
+    g.advance(5.0)
+    out["sheet/traj"] = g.trajectory.copy()
+    out["sheet/in"] = np.array([4.0e5, 40.0, g.mu, 0.5, 5.0, g.tcur])
+    out["sheet/pos"] = np.array([1.0 * Re, 2.0 * Re, 0.9 * Re])
+    reset_params()
+    np.savez_compressed(os.path.join(HERE, "gcs.npz"), **out)
+
+
+def make_utils():
+    out = {}
+    n = 16
+    f = rf.DoubleDipole()
+    pts = sample_points("dipole", n, 21)
+    v = W.speedfromKE(10 ** W.uniform(21, 7, n, 3, 7), rapt.m_el)
+    th = np.arccos(W.uniform(21, 8, n, -1, 1))
+    ph = W.uniform(21, 9, n, 0, 2 * np.pi)
+    vel = v[:, None] * np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], 1)
+    cp, cr, gcs, g2f = [], [], [], []
+    for i in range(n):
+        t, pos = pts[i, 0], list(pts[i, 1:])
+        cp.append(ru.cyclotron_period(t, pos, list(vel[i]), f, rapt.m_el, -rapt.e))
+        cr.append(ru.cyclotron_radius(t, pos, list(vel[i]), f, rapt.m_el, -rapt.e))
+        gc, vp, spd = ru.guidingcenter(t, pos, vel[i], f, rapt.m_el, -rapt.e)
+        gcs.append(list(gc) + [vp, spd])
+        vpar = v[i] * np.cos(th[i])
+        for gp in (0.0, 1.3):
+            p2, v2 = ru.GCtoFP(t, pos, vpar, v[i], f, rapt.m_el, -rapt.e, gp)
+            g2f.append(list(p2) + list(v2))
+    out["tpos"], out["vel"], out["v"], out["th"] = pts, vel, v, th
+    out["cyclotron_period"] = np.array(cp)
+    out["cyclotron_radius"] = np.array(cr)
+    out["guidingcenter"] = np.array(gcs)
+    out["gctofp"] = np.array(g2f)
+    out["mu"] = np.array([ru.magnetic_moment(pts[i, 0], list(pts[i, 1:]), v[i] * np.cos(th[i]),
+                                             v[i], f, rapt.m_el) for i in range(n)])
+    out["speedfromKE"] = np.array([[k, ru.speedfromKE(k, rapt.m_el), ru.speedfromKE(k, rapt.m_pr)]
+                                   for k in (1.0, 1e3, 1e5, 1e6, 1e7)])
+    # adiabaticity decisions (rapt/Particle.py:345-384, rapt/GuidingCenter.py:287-327)
+    reset_params()
+    dec = []
+    fs = RefCurrentSheet()
+    cs = W.current_sheet_protons(n=n)
+    for i in range(n):
+        p = rapt.Particle(list(cs["pos"][i]), list(cs["vel"][i]), 0, rapt.m_pr, rapt.e, fs)
+        dec.append([p.isadiabatic(), p.cycrad(), fs.lengthscale(p.trajectory[-1, :4])])
+    out["sheet_particle_adiabatic"] = np.array(dec, dtype=float)
+    out["sheet_pos"], out["sheet_vel"] = cs["pos"], cs["vel"]
+    np.savez_compressed(os.path.join(HERE, "utils.npz"), **out)
+
+
+def make_rkf():
+    out = {}
+    fn = lambda x, t: np.array([x[1], -x[0]])
+    for j, (tol, hmax, hmin) in enumerate([(1e-6, 0.5, 1e-6), (1e-9, 0.1, 1e-8), (np.inf, 0.05, 1e-6), (np.inf, 0.07, 1e-9)]):
+        raised = 0
+        try:
+            with np.errstate(divide="ignore"):
+                T, X = ref_rkf(fn, 0.0, 3.0, np.array([1.0, 0.0]), tol, hmax, hmin)
+        except RuntimeError:
+            raised = 1
+            T, X = np.zeros(0), np.zeros((0, 2))
+        out["osc%d/T" % j], out["osc%d/X" % j] = T, X
+        out["osc%d/cfg" % j] = np.array([tol, hmax, hmin, raised])
+    np.savez_compressed(os.path.join(HERE, "rkf.npz"), **out)
+
+
+def make_bounce():
+    out = {}
+    rows = []
+    for fname, f in (("DoubleDipole", rf.DoubleDipole()), ("EarthDipole", rf.EarthDipole())):
+        for L in (3.0, 5.0, 7.0):
+            for pa in (20.0, 45.0, 75.0, 89.0):
+                for ph in (0.0, 2.5):
+                    tpos = [0.0, L * Re * np.cos(ph), L * Re * np.sin(ph), 0.0]
+                    B = f.magB(tpos)
+                    Bm = B / np.sin(np.radians(pa)) ** 2
+                    sb = rfu.halfbouncepath(tpos, f, Bm)
+                    rows.append([0 if fname == "DoubleDipole" else 1, L, pa, ph, Bm, sb])
+    out["halfbouncepath"] = np.array(rows)
+    np.savez_compressed(os.path.join(HERE, "bounce.npz"), **out)
+
+
+if __name__ == "__main__":
+    for fn in (make_fields, make_particles, make_gcs, make_utils, make_rkf, make_bounce):
+        t = time.time()
+        fn()
+        print(fn.__name__, "%.1f s" % (time.time() - t), flush=True)
