@@ -77,8 +77,16 @@ def test_field_eval_device_vs_reference(golden, name):
             continue
         if name == "CurrentSheet" and w in ("gradB", "curlb", "jacobianB", "lengthscale"):
             tol = 1e-6  # tanh differs by an ulp; d = lam/1000 amplifies it
-        err = rel_err(got, ref, axis=None)
-        assert err <= tol, (name, w, err)
+        if w == "dbdt":
+            # b(t) of these models has a fixed direction: the reference's value is
+            # rounding noise of order ulp / timederivstepsize
+            np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11, err_msg=w)
+            continue
+        finite = np.isfinite(ref)
+        np.testing.assert_array_equal(np.isfinite(got), finite, err_msg=w)
+        scale = np.max(np.abs(ref[finite])) if finite.any() else 1.0
+        np.testing.assert_allclose(got[finite], ref[finite], rtol=0, atol=tol * scale,
+                                   err_msg="%s.%s" % (name, w))
 
 
 def _golden_particles(golden):
