@@ -13,6 +13,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// No FMA contraction in field and right-hand-side code: finite differences
+// must cancel exactly where the reference's do (a uniform field has zero
+// curl), which fused a*b-c*d breaks.  Products are fused only where written
+// as fma (dot3, mirroring numpy's BLAS dot).
+#pragma clang fp contract(off)
+
 namespace rapt {
 
 constexpr double C_LIGHT = 299792458.0;
@@ -300,3 +306,5 @@ __device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, 
 }
 
 }  // namespace rapt
+
+#pragma clang fp contract(fast)
