@@ -134,12 +134,23 @@ int rapt_field_set(rapt_ctx *ctx, const rapt_field_desc *field);
  * 11 timescale.  tpos: n x 4 host array; out: n x width(what) host array. */
 int rapt_field_eval(rapt_ctx *ctx, int what, int64_t n, const double *tpos, double *out);
 
+/* Half-bounce path length S_b for n guiding centres (flutils.halfbouncepath,
+ * rapt/flutils.py:254-316, as GuidingCenter.bounceperiod uses it,
+ * rapt/GuidingCenter.py:593-606): field-line trace, mirror points, integral.
+ * tpos: n x 4 host array; Bm: mirror fields; sb: out (m); status: 1 ok, < 0
+ * on failure.  Needs a dipole-type field (EarthDipole, DoubleDipole,
+ * VarEarthDipole, EarthDipole + uniform). */
+int rapt_halfbounce(rapt_ctx *ctx, int64_t n, const double *tpos, const double *Bm,
+                    double fieldlineresolution, double *sb, int32_t *status);
+
 /* ---- device-resident batch (replaces one scipy ode loop per particle) -- */
 int rapt_batch_create(rapt_ctx *ctx, int64_t n, rapt_batch **out);
 int rapt_batch_destroy(rapt_batch *b);
 /* state: RAPT_NSLOT x n slot-major host array; flags: per-particle int32
  * (bit 0: advance already started; bits 1-2: adaptive segment mode). */
 int rapt_batch_upload(rapt_batch *b, const double *state, const int32_t *flags);
+/* Restore the state and flags of the last upload (device to device). */
+int rapt_batch_reset(rapt_batch *b);
 /* Row storage: row_cap[i] rows of `width` doubles for particle i (width 7
  * PARTICLE, 5 GC, 8 ADAPTIVE).  Resets the per-particle row counters. */
 int rapt_batch_set_rows(rapt_batch *b, int32_t width, const int64_t *row_cap);

@@ -80,17 +80,21 @@ def lib():
     L.rapt_last_error.restype = C.c_char_p
     L.rapt_field_set.argtypes = [vp, C.POINTER(FieldDesc)]
     L.rapt_field_eval.argtypes = [vp, C.c_int, C.c_int64, dp, dp]
+    L.rapt_halfbounce.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
     L.rapt_batch_create.argtypes = [vp, C.c_int64, C.POINTER(vp)]
     L.rapt_batch_destroy.argtypes = [vp]
     L.rapt_batch_upload.argtypes = [vp, dp, i32p]
     L.rapt_batch_set_rows.argtypes = [vp, C.c_int32, i64p]
+    L.rapt_batch_reset.argtypes = [vp]
     L.rapt_batch_run.argtypes = [vp, C.POINTER(TraceDesc), C.POINTER(Counters)]
     L.rapt_batch_download.argtypes = [vp, dp, i32p, i32p, i64p, dp, i64p]
     L.rapt_trace.argtypes = [vp, C.POINTER(TraceDesc), C.c_int64, dp, i32p, i64p, dp, i64p,
                              i32p, C.POINTER(Counters)]
     for nm in ("rapt_ctx_create", "rapt_ctx_destroy", "rapt_field_set", "rapt_field_eval",
+               "rapt_halfbounce",
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
-               "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace"):
+               "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
+               "rapt_batch_reset"):
         getattr(L, nm).restype = C.c_int
     _lib = L
     return L
@@ -152,6 +156,20 @@ class Context:
         return out
 
 
+    def halfbounce(self, desc, tpos, Bm, fieldlineresolution):
+        """S_b per particle on the device; returns (sb, status)."""
+        self.set_field(desc)
+        tpos = np.ascontiguousarray(np.atleast_2d(tpos), dtype=np.float64)
+        n = tpos.shape[0]
+        Bm = np.ascontiguousarray(np.broadcast_to(Bm, (n,)), dtype=np.float64)
+        sb = np.zeros(n)
+        st = np.zeros(n, np.int32)
+        self.check(lib().rapt_halfbounce(self.h, n, _dp(tpos), _dp(Bm),
+                                         float(fieldlineresolution), _dp(sb), _ip32(st)),
+                   "rapt_halfbounce")
+        return sb, st
+
+
 def context(device=0):
     with _lock:
         if device not in _ctx:
@@ -187,6 +205,10 @@ class Batch:
         assert state.shape == (NSLOT, self.n) and flags.shape == (self.n,)
         self.ctx.check(lib().rapt_batch_upload(self.h, _dp(state), _ip32(flags)),
                        "rapt_batch_upload")
+
+    def reset(self):
+        """Restore the uploaded state on the device (no host transfer)."""
+        self.ctx.check(lib().rapt_batch_reset(self.h), "rapt_batch_reset")
 
     def set_rows(self, width, row_cap):
         row_cap = np.ascontiguousarray(row_cap, dtype=np.int64)

@@ -139,6 +139,41 @@ def particle_dt(field, last7, mass, charge, cyclotronresolution, device=0):
     return 2 * np.pi * g * m / B / np.abs(charge) / cyclotronresolution
 
 
+def gc_mirror_speed(field, last5, mass, mu, device=0):
+    """Mirror field and speed of guiding centres, as GuidingCenter.bounceperiod
+    computes them (rapt/GuidingCenter.py:593-606), with |B| from the device."""
+    last5 = np.atleast_2d(last5)
+    n = last5.shape[0]
+    m = np.broadcast_to(np.asarray(mass, float), (n,))
+    mu = np.broadcast_to(np.asarray(mu, float), (n,))
+    B = N.context(device).field_eval(field.descriptor(), "magB", last5[:, :4])[:, 0]
+    pp = last5[:, 4]
+    g = np.sqrt(1 + 2 * mu * B / (m * c * c) + (pp / (m * c)) ** 2)
+    nr = g - 1 < 1e-6
+    p_nr = np.sqrt(2 * m * mu * B + pp ** 2)
+    p_r = m * c * np.sqrt((g + 1) * (g - 1))
+    p = np.where(nr, p_nr, p_r)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        Bmir = np.where(nr, p_nr ** 2 / (2 * m * mu), p_r ** 2 / ((p_r - pp) * (p_r + pp)) * B)
+    v = np.where(nr, p / m, p / m / g)
+    return Bmir, v
+
+
+def gc_bounce_dt(field, last5, mass, mu, params=None, device=0):
+    """Output step bounceperiod()/bounceresolution for a batch of guiding
+    centres, with the half-bounce path computed on the device."""
+    P = _params(params)
+    last5 = np.atleast_2d(np.asarray(last5, dtype=np.float64))
+    Bmir, v = gc_mirror_speed(field, last5, mass, mu, device)
+    sb, st = N.context(device).halfbounce(field.descriptor(), last5[:, :4], Bmir,
+                                          P["fieldlineresolution"])
+    if np.any(st != 1):
+        bad = np.flatnonzero(st != 1)
+        raise RuntimeError("bounce period failed for %d particle(s), first %d (status %d)"
+                           % (bad.size, bad[0], st[bad[0]]))
+    return 2.0 / v * sb / P["bounceresolution"], sb
+
+
 def advance_particles(field, last7, mass, charge, delta, params=None, check=False,
                       stepper=None, row_cap=None, lanes=0, device=0, **kw):
     """Particle.advance for a batch.  last7: (n, 7) rows (t, x, y, z, px, py, pz)."""

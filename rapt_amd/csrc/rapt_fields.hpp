@@ -192,6 +192,21 @@ __device__ __forceinline__ double lengthscale(const FieldC& F, double t, double 
   return magB<M>(F, t, x, y, z) / jacobian_maxabs<M>(F, t, x, y, z);
 }
 
+// _Field.curvature (rapt/fields.py:155-174) as written there: np.dot(gB, |B|)
+// with a scalar |B| is an elementwise product, so the "perpendicular" part is
+// gB_i (1 - B_i/|B|)
+template <int M>
+__device__ __forceinline__ double curvature(const FieldC& F, double t, double x, double y,
+                                            double z) {
+  const V3 B = fieldB<M>(F, t, x, y, z);
+  const double Bm = __builtin_sqrt(dot3(B, B));
+  V3 g, c;
+  grad_curl<M>(F, t, x, y, z, g, c);
+  const double s = Bm / (Bm * Bm);
+  const V3 gp{g.x - (g.x * s) * B.x, g.y - (g.y * s) * B.y, g.z - (g.z * s) * B.z};
+  return __builtin_sqrt(dot3(gp, gp)) / Bm;
+}
+
 // _Field.timescale (rapt/fields.py:263-280) for non-static fields
 template <int M>
 __device__ __forceinline__ double timescale(const FieldC& F, double t, double x, double y,

@@ -33,6 +33,7 @@
 
 #include "../../include/rapt_hip.h"
 #include "rapt_fields.hpp"
+#include "rapt_internal.hpp"
 #include "rapt_steppers.hpp"
 
 namespace rapt {
@@ -672,16 +673,7 @@ __global__ __launch_bounds__(256) void k_field_eval(FieldC F, int what, int64_t 
       }
       break;
     }
-    case 6: {
-      // _Field.curvature (rapt/fields.py:155-174), elementwise as written there
-      V3 B = fieldB<M>(F, t, x, y, z), g, c;
-      const double Bm = __builtin_sqrt(dot3(B, B));
-      grad_curl<M>(F, t, x, y, z, g, c);
-      const double s = Bm / (Bm * Bm);
-      const V3 gp{g.x - (g.x * s) * B.x, g.y - (g.y * s) * B.y, g.z - (g.z * s) * B.z};
-      o[0] = __builtin_sqrt(dot3(gp, gp)) / Bm;
-      break;
-    }
+    case 6: o[0] = curvature<M>(F, t, x, y, z); break;
     case 8: {
       if (F.is_static) {
         o[0] = 0.0;
@@ -710,19 +702,12 @@ __global__ __launch_bounds__(256) void k_field_eval(FieldC F, int what, int64_t 
 // ============================================================================
 using namespace rapt;
 
-struct rapt_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  FieldC field{};
-  bool field_set = false;
-  std::string err;
-};
-
 struct rapt_batch {
   rapt_ctx* ctx = nullptr;
   int64_t n = 0;
   double* d_state = nullptr;
+  double* d_state0 = nullptr;  // state as uploaded (rapt_batch_reset)
+  int32_t* d_flags0 = nullptr;
   int32_t* d_flags = nullptr;
   int32_t* d_status = nullptr;
   int64_t* d_nrows = nullptr;
@@ -735,24 +720,6 @@ struct rapt_batch {
   unsigned long long* d_work = nullptr;
 };
 
-static int set_err(rapt_ctx* c, int code, const char* fmt, ...) {
-  if (c) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    c->err = buf;
-  }
-  return code;
-}
-
-#define HIP_OK(ctx, call)                                                              \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return set_err((ctx), RAPT_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
 
 static int field_width(int what) {
   switch (what) {
@@ -904,6 +871,8 @@ int rapt_batch_create(rapt_ctx* c, int64_t n, rapt_batch** out) {
   b->ctx = c;
   b->n = n;
   bool ok = hipMalloc(&b->d_state, sizeof(double) * RAPT_NSLOT * n) == hipSuccess &&
+            hipMalloc(&b->d_state0, sizeof(double) * RAPT_NSLOT * n) == hipSuccess &&
+            hipMalloc(&b->d_flags0, sizeof(int32_t) * n) == hipSuccess &&
             hipMalloc(&b->d_flags, sizeof(int32_t) * n) == hipSuccess &&
             hipMalloc(&b->d_status, sizeof(int32_t) * n) == hipSuccess &&
             hipMalloc(&b->d_nrows, sizeof(int64_t) * n) == hipSuccess &&
@@ -924,6 +893,8 @@ int rapt_batch_destroy(rapt_batch* b) {
   if (!b) return RAPT_E_ARG;
   if (b->ctx) (void)hipSetDevice(b->ctx->device);
   (void)hipFree(b->d_state);
+  (void)hipFree(b->d_state0);
+  (void)hipFree(b->d_flags0);
   (void)hipFree(b->d_flags);
   (void)hipFree(b->d_status);
   (void)hipFree(b->d_nrows);
@@ -944,7 +915,22 @@ int rapt_batch_upload(rapt_batch* b, const double* state, const int32_t* flags) 
                            hipMemcpyHostToDevice, c->stream));
   HIP_OK(c, hipMemcpyAsync(b->d_flags, flags, sizeof(int32_t) * b->n, hipMemcpyHostToDevice,
                            c->stream));
+  HIP_OK(c, hipMemcpyAsync(b->d_state0, b->d_state, sizeof(double) * RAPT_NSLOT * b->n,
+                           hipMemcpyDeviceToDevice, c->stream));
+  HIP_OK(c, hipMemcpyAsync(b->d_flags0, b->d_flags, sizeof(int32_t) * b->n,
+                           hipMemcpyDeviceToDevice, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
+  return RAPT_OK;
+}
+
+int rapt_batch_reset(rapt_batch* b) {
+  if (!b) return RAPT_E_ARG;
+  rapt_ctx* c = b->ctx;
+  HIP_OK(c, hipSetDevice(c->device));
+  HIP_OK(c, hipMemcpyAsync(b->d_state, b->d_state0, sizeof(double) * RAPT_NSLOT * b->n,
+                           hipMemcpyDeviceToDevice, c->stream));
+  HIP_OK(c, hipMemcpyAsync(b->d_flags, b->d_flags0, sizeof(int32_t) * b->n,
+                           hipMemcpyDeviceToDevice, c->stream));
   return RAPT_OK;
 }
 
