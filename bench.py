@@ -87,12 +87,25 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the tracer shards particles with no exchange, "
               "and this build runs and reports a single MI355X only (--gpus 1).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe // 4, dtype=torch.float32, device="cuda")
+        a.fill_(1.0)
+        bb = torch.empty_like(a)
+        torch.cuda.synchronize()
+        bb.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}))
+        return
 
     from rapt_amd import _native as N
     w = build_workload(args.n)
@@ -129,6 +142,9 @@ def main():
     el = time.perf_counter() - t0
     out = b.download(rows=False)
     ok = int(np.sum(out["status"] == 1))
+    q = np.percentile(out["naccpt"], [50, 90, 99, 100])
+    log("accepted steps per particle: mean %.0f, p50 %.0f, p90 %.0f, p99 %.0f, max %.0f"
+        % (out["naccpt"].mean(), *q))
     value = tot["naccpt"] / el
     launches = args.steps
     flop_launch = (tot["nfcn"] * FLOP_RHS_TCB_DOUBLEDIPOLE +

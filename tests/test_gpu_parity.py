@@ -318,7 +318,7 @@ def test_halfbounce_device_vs_reference(golden):
     """Device half-bounce path S_b (field-line trace, spline, mirror points,
     integral) against flutils.halfbouncepath of the reference, 48 cases in
     the DoubleDipole and EarthDipole fields.  The reference integrates with
-    QUADPACK at epsrel 1e-4; the agreement bar is 1e-5."""
+    QUADPACK at epsrel 1e-4, which is also the agreement bar here."""
     rows = golden("bounce")["halfbouncepath"]
     ctx = N.context()
     for fid in (0, 1):
@@ -329,4 +329,5 @@ def test_halfbounce_device_vs_reference(golden):
                      np.zeros(len(L))]
         got, st = ctx.halfbounce(f.descriptor(), tpos, Bm, rapt.params["fieldlineresolution"])
         assert np.all(st == 1), st
-        np.testing.assert_allclose(got, sb, rtol=1e-5)
+        np.testing.assert_allclose(got, sb, rtol=1e-4)
+        assert np.median(np.abs(got / sb - 1)) < 1e-6
