@@ -192,95 +192,52 @@ __device__ __forceinline__ double dop853_attempt(RHS& f, double x, double h, con
                                                  const double* k1, double* y1, double atol,
                                                  double rtol) {
   using namespace dp8;
-  double k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], k8[N], k9[N], k10[N], t[N], kk[N];
-  // stages 2..12 through one right-hand-side call site (wave-uniform stage index)
-#pragma unroll 1
-  for (int s = 2; s <= 12; ++s) {
-    double xs;
-    switch (s) {
-      case 2:
+  double k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], k8[N], k9[N], k10[N], t[N];
 #pragma unroll
-        for (int i = 0; i < N; ++i) t[i] = y[i] + h * a21 * k1[i];
-        xs = x + c2 * h;
-        break;
-      case 3:
+  for (int i = 0; i < N; ++i) t[i] = y[i] + h * a21 * k1[i];
+  f(x + c2 * h, t, k2);
 #pragma unroll
-        for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
-        xs = x + c3 * h;
-        break;
-      case 4:
+  for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
+  f(x + c3 * h, t, k3);
 #pragma unroll
-        for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a41 * k1[i] + a43 * k3[i]);
-        xs = x + c4 * h;
-        break;
-      case 5:
+  for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a41 * k1[i] + a43 * k3[i]);
+  f(x + c4 * h, t, k4);
 #pragma unroll
-        for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
-        xs = x + c5 * h;
-        break;
-      case 6:
+  for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
+  f(x + c5 * h, t, k5);
 #pragma unroll
-        for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
-        xs = x + c6 * h;
-        break;
-      case 7:
+  for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
+  f(x + c6 * h, t, k6);
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-          t[i] = y[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
-        xs = x + c7 * h;
-        break;
-      case 8:
+  for (int i = 0; i < N; ++i)
+    t[i] = y[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
+  f(x + c7 * h, t, k7);
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-          t[i] = y[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] +
-                             a87 * k7[i]);
-        xs = x + c8 * h;
-        break;
-      case 9:
+  for (int i = 0; i < N; ++i)
+    t[i] = y[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] + a87 * k7[i]);
+  f(x + c8 * h, t, k8);
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-          t[i] = y[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] +
-                             a97 * k7[i] + a98 * k8[i]);
-        xs = x + c9 * h;
-        break;
-      case 10:
+  for (int i = 0; i < N; ++i)
+    t[i] = y[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i] +
+                       a98 * k8[i]);
+  f(x + c9 * h, t, k9);
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-          t[i] = y[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
-                             a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
-        xs = x + c10 * h;
-        break;
-      case 11:
+  for (int i = 0; i < N; ++i)
+    t[i] = y[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
+                       a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
+  f(x + c10 * h, t, k10);
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-          t[i] = y[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
-                             a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
-        xs = x + c11 * h;
-        break;
-      default:
+  for (int i = 0; i < N; ++i)
+    t[i] = y[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
+                       a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
+  f(x + c11 * h, t, k2);  // stage 11 overwrites k2, as in Hairer's code
+  const double xph = x + h;
 #pragma unroll
-        for (int i = 0; i < N; ++i)
-          t[i] = y[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
-                             a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
-                             a1211 * k2[i]);
-        xs = x + h;
-        break;
-    }
-    f(xs, t, kk);
-    // stage 11 overwrites k2 and stage 12 k3, as in Hairer's code
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      if (s == 2 || s == 11) k2[i] = kk[i];
-      else if (s == 3 || s == 12) k3[i] = kk[i];
-      else if (s == 4) k4[i] = kk[i];
-      else if (s == 5) k5[i] = kk[i];
-      else if (s == 6) k6[i] = kk[i];
-      else if (s == 7) k7[i] = kk[i];
-      else if (s == 8) k8[i] = kk[i];
-      else if (s == 9) k9[i] = kk[i];
-      else k10[i] = kk[i];
-    }
-  }
+  for (int i = 0; i < N; ++i)
+    t[i] = y[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
+                       a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
+                       a1211 * k2[i]);
+  f(xph, t, k3);  // stage 12 overwrites k3
   double err = 0.0, err2 = 0.0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
