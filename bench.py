@@ -31,6 +31,10 @@ FLOP_RHS_TCB_REFERENCE = 795         # same RHS as the reference writes it (13 f
 PEAK_FP64 = 78.6e12                  # MI355X FP64 vector (and matrix) peak, FLOP/s
 PEAK_HBM = 8.0e12                    # bytes/s
 ROW_BYTES_GC = 40                    # one (t, x, y, z, ppar) output row
+# HBM bytes per k_trace launch at the default batch, from rocprofv3 FETCH_SIZE
+# and WRITE_SIZE (KiB) scaled by a 256 MiB copy probe (read x2.00, write x1.00);
+# profiles/k_trace_gc_doubledipole_65536.md has the commands and raw values
+PROFILED_TRAFFIC = {65536: (5172.6875 * 2.0 + 3057523.5 * 1.0) * 1024}
 
 
 def log(*a):
@@ -180,7 +184,7 @@ def main():
         "roofline": {
             "bound": "mfma", "compute": "FP64 vector ALU (78.6 TFLOP/s, same as FP64 matrix)",
             "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64, "traffic": None,
+            "frac": achieved / PEAK_FP64, "traffic": PROFILED_TRAFFIC.get(n),
             "flop_per_launch": flop_launch,
             "flop_per_particle_step": flop_launch / (tot["naccpt"] / launches),
             "reference_formulation_flop_per_launch":
