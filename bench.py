@@ -89,8 +89,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--lpp", type=int, default=4, choices=(1, 4),
+                    help="lanes per guiding centre (4: one lane quad per particle)")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--order", choices=("input", "dt"), default="dt",
+                    help="library work order: shortest output step (bounce period) first "
+                         "(default), or input order")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -120,7 +125,8 @@ def main():
     S = BT.gc_state(w["last5"], w["m"], w["q"], w["mu"], w["v"], w["dt"])
     flags = np.zeros(n, np.int32)
     cap = BT._rows_needed(w["delta"], w["dt"])
-    desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes)
+    desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes,
+                       lanes_per_particle=args.lpp, work_order=0 if args.order == "dt" else 1)
     b = N.Batch(ctx, n)
     b.upload(S, flags)
     b.set_rows(5, cap)
@@ -174,7 +180,8 @@ def main():
                         "L U[3,8], eq. pitch U[15,85] deg; TCB EOM, DOPRI5 rtol=atol=1e-8, "
                         "advance(60), dt=bounceperiod/10" % n,
             "particles": n, "delta_s": 60.0, "rtol": 1e-8, "atol": 1e-8,
-            "lanes": args.lanes or n,
+            "lanes": args.lanes or n * args.lpp, "lanes_per_particle": args.lpp,
+            "order": args.order,
         },
         "particles_done": ok,
         "accepted_steps_per_step": tot["naccpt"] / launches,

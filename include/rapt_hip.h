@@ -87,6 +87,11 @@ typedef struct {
     double delta;                /* advance(delta) */
     int64_t lanes;               /* resident lanes of the persistent grid (0 = auto);
                                     fewer lanes than particles turns on work refill */
+    int32_t lanes_per_particle;  /* GC with DOPRI5: 4 (default, 0) spreads one guiding
+                                    centre over a lane quad; 1 = one lane per particle */
+    int32_t work_order;          /* 0: particles with the shortest output step (slot
+                                    RAPT_S_DT at upload) first; 1: input order.  Results
+                                    per particle do not depend on it */
 } rapt_trace_desc;
 
 /* Per-particle state, slot-major (state[slot * n + i]).  Filled by the host

@@ -11,7 +11,9 @@ import threading
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "librapt_hip.so")
+# RAPT_HIP_LIB names another build of the same library (e.g. one compiled
+# with different launch bounds)
+LIB_PATH = os.environ.get("RAPT_HIP_LIB") or os.path.join(_HERE, "lib", "librapt_hip.so")
 
 MODE_PARTICLE, MODE_GC, MODE_ADAPTIVE = 0, 1, 2
 EOM = {"TaoChanBrizardEOM": 0, "BrizardChanEOM": 1, "NorthropTellerEOM": 2}
@@ -49,7 +51,8 @@ class TraceDesc(C.Structure):
                 ("rkf_hmax", C.c_double), ("rkf_hmin", C.c_double),
                 ("cyclotronresolution", C.c_double), ("gctimestep", C.c_double),
                 ("epss", C.c_double), ("epst", C.c_double), ("delta", C.c_double),
-                ("lanes", C.c_int64)]
+                ("lanes", C.c_int64), ("lanes_per_particle", C.c_int32),
+                ("work_order", C.c_int32)]
 
 
 class Counters(C.Structure):
@@ -277,4 +280,6 @@ def make_desc(mode, params, delta, eom="TaoChanBrizardEOM", stepper=None, check=
     d.epst = float(params["epst"])
     d.delta = float(delta)
     d.lanes = int(lanes)
+    d.lanes_per_particle = int(kw.get("lanes_per_particle", 0))
+    d.work_order = int(kw.get("work_order", 0))
     return d

@@ -13,10 +13,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-// No FMA contraction in field and right-hand-side code: finite differences
-// must cancel exactly where the reference's do (a uniform field has zero
-// curl), which fused a*b-c*d breaks.  Products are fused only where written
-// as fma (dot3, mirroring numpy's BLAS dot).
+// No FMA contraction anywhere after this header (the library is also built
+// with -ffp-contract=off): finite differences must cancel exactly where the
+// reference's do (a uniform field has zero curl), which fused a*b-c*d breaks,
+// and the one-lane and four-lane guiding-centre kernels must round every
+// expression the same way to give identical trajectories.  Products are
+// fused only where written as fma (dot3, mirroring numpy's BLAS dot, and the
+// sums of squares of the error norms).
 #pragma clang fp contract(off)
 
 namespace rapt {
@@ -154,6 +157,13 @@ __device__ __forceinline__ double magB(const FieldC& F, double t, double x, doub
   return __builtin_sqrt(dot3(B, B));
 }
 
+// curl b from the differences of b at opposite stencil points:
+// dxy = b_y(+x) - b_y(-x), and so on
+__device__ __forceinline__ V3 curl_from_pairs(double dxy, double dxz, double dyx, double dyz,
+                                              double dzx, double dzy, double i2d) {
+  return V3{(dyz - dzy) * i2d, (dzx - dxz) * i2d, (dxy - dyx) * i2d};
+}
+
 // Stencil of _Field.gradB and _Field.curlb (rapt/fields.py:111-131, 176-200)
 // from one field evaluation per point.  Points: +x, -x, +y, -y, +z, -z.
 template <int M>
@@ -176,11 +186,10 @@ __device__ __forceinline__ void grad_curl(const FieldC& F, double t, double x, d
     u[k] = V3{B.x * ib, B.y * ib, B.z * ib};
   }
   gB = V3{(m[0] - m[1]) * i2d, (m[2] - m[3]) * i2d, (m[4] - m[5]) * i2d};
-  // the _M1 rows (rapt/fields.py:33-36), accumulated as numpy's gemv does
-  const double r0 = (u[2].z + (-u[3].z - u[4].y)) + u[5].y;
-  const double r1 = (-u[0].z + u[4].x) + (u[1].z - u[5].x);
-  const double r2 = (-u[1].y - u[2].x) + (u[0].y + u[3].x);
-  cb = V3{r0 * i2d, r1 * i2d, r2 * i2d};
+  // the _M1 rows (rapt/fields.py:33-36) as differences of opposite points;
+  // the four-lane kernel forms the same differences in separate lanes
+  cb = curl_from_pairs(u[0].y - u[1].y, u[0].z - u[1].z, u[2].x - u[3].x, u[2].z - u[3].z,
+                       u[4].x - u[5].x, u[4].y - u[5].y, i2d);
 }
 
 // _Field.dbdt (rapt/fields.py:225-245); zero for static fields
@@ -349,5 +358,3 @@ __device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, 
 }
 
 }  // namespace rapt
-
-#pragma clang fp contract(fast)

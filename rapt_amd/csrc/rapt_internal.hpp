@@ -9,6 +9,44 @@
 #include "../../include/rapt_hip.h"
 #include "rapt_fields.hpp"
 
+namespace rapt {
+
+constexpr int KIND_PARTICLE = 0;
+constexpr int KIND_GC = 1;
+constexpr int KIND_ADAPTIVE = 2;
+
+constexpr int FLAG_STARTED = 1;   // advance() in progress (state slots valid)
+constexpr int FLAG_GCMODE = 2;    // ADAPTIVE: current segment is a GuidingCenter
+constexpr int FLAG_AINIT = 4;     // ADAPTIVE: initial mode not decided yet
+
+struct CtlParams {  // scipy set_integrator knobs of one stepper
+  double safe, facc1, facc2, beta, expo1;
+};
+
+struct KArgs {
+  FieldC F;
+  int32_t kind, eom, stepper, nmax;
+  int32_t eq, check, width, pad;
+  double rtol, atol, gc_rtol, gc_atol;
+  CtlParams c853, c5;
+  double max_step, first_step;
+  double rkf_tol, rkf_hmax, rkf_hmin;
+  double cycres, gcdt, epss, epst, delta;
+  int64_t n;
+  double* state;
+  int32_t* flags;
+  int32_t* status;
+  int64_t* nrows;
+  int64_t* naccpt;
+  const int64_t* row_off;
+  const int64_t* row_cap;
+  double* rows;
+  unsigned long long* work;  // [0] next work item, [1..6] counters
+  const int64_t* order;      // work item -> particle index (nullptr: identity)
+};
+
+}  // namespace rapt
+
 struct rapt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;

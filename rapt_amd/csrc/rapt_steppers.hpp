@@ -102,7 +102,7 @@ __device__ __forceinline__ double dopri5_attempt(RHS& f, double x, double h, con
                       e7 * k7[i]) * h;
     const double sk = atol + rtol * fmax(__builtin_fabs(y[i]), __builtin_fabs(y1[i]));
     const double q = e / sk;
-    err += q * q;
+    err = __builtin_fma(q, q, err);
   }
   return __builtin_sqrt(err * (1.0 / N));
 }
@@ -267,8 +267,8 @@ __device__ __forceinline__ double hinit(RHS& f, double x, const double* y, const
   for (int i = 0; i < N; ++i) {
     sk[i] = atol + rtol * __builtin_fabs(y[i]);
     const double a = f0[i] / sk[i], b = y[i] / sk[i];
-    dnf += a * a;
-    dny += b * b;
+    dnf = __builtin_fma(a, a, dnf);
+    dny = __builtin_fma(b, b, dny);
   }
   double h = (dnf <= 1.0e-10 || dny <= 1.0e-10) ? 1.0e-6 : __builtin_sqrt(dny / dnf) * 0.01;
   h = fmin(h, hmax);
@@ -280,7 +280,7 @@ __device__ __forceinline__ double hinit(RHS& f, double x, const double* y, const
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const double q = (f1[i] - f0[i]) / sk[i];
-    der2 += q * q;
+    der2 = __builtin_fma(q, q, der2);
   }
   der2 = __builtin_sqrt(der2) / h;
   const double der12 = fmax(__builtin_fabs(der2), __builtin_sqrt(dnf));

@@ -28,48 +28,19 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <math.h>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/rapt_hip.h"
 #include "rapt_fields.hpp"
 #include "rapt_internal.hpp"
+#include "rapt_physics.hpp"
 #include "rapt_steppers.hpp"
 
 namespace rapt {
-
-constexpr int KIND_PARTICLE = 0;
-constexpr int KIND_GC = 1;
-constexpr int KIND_ADAPTIVE = 2;
-
-constexpr int FLAG_STARTED = 1;   // advance() in progress (state slots valid)
-constexpr int FLAG_GCMODE = 2;    // ADAPTIVE: current segment is a GuidingCenter
-constexpr int FLAG_AINIT = 4;     // ADAPTIVE: initial mode not decided yet
-
-struct CtlParams {  // scipy set_integrator knobs of one stepper
-  double safe, facc1, facc2, beta, expo1;
-};
-
-struct KArgs {
-  FieldC F;
-  int32_t kind, eom, stepper, nmax;
-  int32_t eq, check, width, pad;
-  double rtol, atol, gc_rtol, gc_atol;
-  CtlParams c853, c5;
-  double max_step, first_step;
-  double rkf_tol, rkf_hmax, rkf_hmin;
-  double cycres, gcdt, epss, epst, delta;
-  int64_t n;
-  double* state;
-  int32_t* flags;
-  int32_t* status;
-  int64_t* nrows;
-  int64_t* naccpt;
-  const int64_t* row_off;
-  const int64_t* row_cap;
-  double* rows;
-  unsigned long long* work;  // [0] next particle, [1..6] counters
-};
 
 // ---- right-hand-side functors --------------------------------------------------
 template <int M>
@@ -96,169 +67,6 @@ struct GcRHS {
     ++nf;
   }
 };
-
-// ---- adiabaticity tests ------------------------------------------------------------
-// Particle.isadiabatic (rapt/Particle.py:345-384) on the row (t, r, p)
-template <int M>
-__device__ bool particle_isadiabatic(const FieldC& F, const KArgs& A, double t, const double* y,
-                                     double mass, double charge) {
-  const V3 p{y[3], y[4], y[5]};
-  const double gm = __builtin_sqrt(mass * mass + dot3(p, p) / C_LIGHT_SQ);
-  const V3 v{p.x / gm, p.y / gm, p.z / gm};
-  const double vsq = dot3(v, v);
-  const double gamma = 1.0 / __builtin_sqrt(1.0 - vsq / C_LIGHT_SQ);
-  const V3 B = fieldB<M>(F, t, y[0], y[1], y[2]);
-  const double Bmag = __builtin_sqrt(dot3(B, B));
-  const double vpar = dot3(v, B) / Bmag;
-  const double vperp = __builtin_sqrt(vsq - vpar * vpar);
-  const double rc = gamma * mass * vperp / (__builtin_fabs(charge) * Bmag);
-  bool ok = rc / lengthscale<M>(F, t, y[0], y[1], y[2]) < A.epss;
-  if (!F.is_static && ok) {
-    const double cp = TWO_PI * gamma * mass / Bmag / __builtin_fabs(charge);
-    ok = cp / timescale<M>(F, t, y[0], y[1], y[2]) < A.epst;
-  }
-  return ok;
-}
-
-// GuidingCenter.isadiabatic (rapt/GuidingCenter.py:287-327); the cycper()
-// term keeps the reference's unnormalised ppar**2 (rapt/GuidingCenter.py:535)
-template <int M>
-__device__ bool gc_isadiabatic(const FieldC& F, const KArgs& A, double t, const double* y,
-                               double mass, double charge, double mu) {
-  const double Bmag = magB<M>(F, t, y[0], y[1], y[2]);
-  const double pp = y[3];
-  const double mcc = mass * C_LIGHT * C_LIGHT;
-  const double pr = pp / mass / C_LIGHT;
-  const double gamma = __builtin_sqrt(1.0 + 2.0 * mu * Bmag / mcc + pr * pr);
-  double vp, v;
-  if (gamma - 1.0 < 1e-6) {
-    vp = pp / mass;
-    v = __builtin_sqrt(2.0 * mu * Bmag / mass + vp * vp);
-  } else {
-    vp = pp / mass / gamma;
-    v = C_LIGHT * __builtin_sqrt(1.0 - 1.0 / (gamma * gamma));
-  }
-  const double g2 = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
-  const double vperp = __builtin_sqrt((v - vp) * (v + vp));
-  const double rc = g2 * mass * vperp / (__builtin_fabs(charge) * Bmag);
-  bool ok = rc / lengthscale<M>(F, t, y[0], y[1], y[2]) < A.epss;
-  if (!F.is_static && ok) {
-    const double g3 = __builtin_sqrt(1.0 + 2.0 * mu * Bmag / mcc + pp * pp);
-    double v3;
-    if (g3 - 1.0 < 1e-6) {
-      const double vp3 = pp / mass;
-      v3 = __builtin_sqrt(2.0 * mu * Bmag / mass + vp3 * vp3);
-    } else {
-      v3 = C_LIGHT * __builtin_sqrt(1.0 - 1.0 / (g3 * g3));
-    }
-    const double g4 = 1.0 / __builtin_sqrt(1.0 - (v3 / C_LIGHT) * (v3 / C_LIGHT));
-    const double cp = TWO_PI * g4 * mass / Bmag / __builtin_fabs(charge);
-    ok = cp / timescale<M>(F, t, y[0], y[1], y[2]) < A.epst;
-  }
-  return ok;
-}
-
-// ---- mode conversions (ADAPTIVE) -----------------------------------------------------
-// utils.guidingcenter (rapt/utils.py:251-326) + GuidingCenter.init(Particle)
-// (rapt/GuidingCenter.py:168-186, __init__ :123-133).  y: Particle state in,
-// GC state out (x, y, z, ppar); returns false when the iteration fails.
-template <int M>
-__device__ bool particle_to_gc(const FieldC& F, double t, double* y, double mass, double charge,
-                               double& mu, double& vout) {
-  const V3 p{y[3], y[4], y[5]};
-  const double gm = __builtin_sqrt(mass * mass + dot3(p, p) / C_LIGHT_SQ);
-  const V3 v{p.x / gm, p.y / gm, p.z / gm};
-  const double vsq = dot3(v, v);
-  const double gamma = 1.0 / __builtin_sqrt(1.0 - vsq / C_LIGHT_SQ);
-  const V3 r{y[0], y[1], y[2]};
-  auto gyro = [&](const V3& q) {
-    const V3 B = fieldB<M>(F, t, q.x, q.y, q.z);
-    const double s = gamma * mass / (charge * dot3(B, B));
-    const V3 c = cross3(B, v);
-    return V3{s * c.x, s * c.y, s * c.z};
-  };
-  V3 g = gyro(r);
-  V3 gold{r.x - g.x, r.y - g.y, r.z - g.z};
-  for (int it = 1; it <= 20; ++it) {
-    g = gyro(gold);
-    const V3 gc{r.x - g.x, r.y - g.y, r.z - g.z};
-    const V3 dd{gc.x - gold.x, gc.y - gold.y, gc.z - gold.z};
-    if (__builtin_sqrt(dot3(dd, dd)) / __builtin_sqrt(dot3(gc, gc)) < 1e-3) {
-      const V3 B = fieldB<M>(F, t, gc.x, gc.y, gc.z);
-      const double Bn = __builtin_sqrt(dot3(B, B));
-      const double vp = dot3(v, B) / Bn;
-      const double spd = __builtin_sqrt(vsq);
-      const double g2 = 1.0 / __builtin_sqrt(1.0 - (spd / C_LIGHT) * (spd / C_LIGHT));
-      const double ppar = mass * g2 * vp;
-      const double vpar = ppar / (mass * g2);
-      mu = g2 * g2 * mass * (spd - vpar) * (spd + vpar) / (2.0 * Bn);
-      vout = spd;
-      y[0] = gc.x;
-      y[1] = gc.y;
-      y[2] = gc.z;
-      y[3] = ppar;
-      y[4] = y[5] = 0.0;
-      return true;
-    }
-    gold = gc;
-  }
-  return false;
-}
-
-// Particle.init(GuidingCenter) (rapt/Particle.py:149-164) with utils.GCtoFP
-// (rapt/utils.py:376-433) at gyrophase 0; the field is read at t = 0 there,
-// the new Particle's tcur.  y: GC state in, Particle state out.
-template <int M>
-__device__ void gc_to_particle(const FieldC& F, double t, double* y, double mass, double charge,
-                               double mu) {
-  (void)t;
-  const double Bm = magB<M>(F, t, y[0], y[1], y[2]);
-  const double pp = y[3];
-  const double pr = pp / mass / C_LIGHT;
-  const double gammasq = 1.0 + 2.0 * mu * Bm / (mass * C_LIGHT * C_LIGHT) + pr * pr;
-  double v;
-  if (__builtin_sqrt(gammasq) - 1.0 < 1e-6) {
-    const double q = pp / mass;
-    v = __builtin_sqrt(2.0 * mu * Bm / mass + q * q);
-  } else {
-    v = C_LIGHT * __builtin_sqrt(1.0 - 1.0 / gammasq);
-  }
-  const double vpar = pp / mass / __builtin_sqrt(gammasq);
-  const V3 B = fieldB<M>(F, 0.0, y[0], y[1], y[2]);
-  const double Bn = __builtin_sqrt(dot3(B, B));
-  const V3 b{B.x / Bn, B.y / Bn, B.z / Bn};
-  const double pa = acos(vpar / v);
-  const double g2 = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
-  const double rc = g2 * mass * __builtin_sqrt((v - vpar) * (v + vpar)) /
-                    (__builtin_fabs(charge) * Bn);
-  V3 u;
-  if (B.x == 0.0) {
-    u = V3{1.0, 0.0, 0.0};
-  } else if (B.y == 0.0) {
-    u = V3{0.0, 1.0, 0.0};
-  } else if (B.z == 0.0) {
-    u = V3{0.0, 0.0, 1.0};
-  } else {
-    const double cc = -1.0 * (B.x + B.y) / B.z;
-    const double nn = __builtin_sqrt(2.0 + cc * cc);
-    u = V3{1.0 / nn, 1.0 / nn, cc / nn};
-  }
-  const double un = __builtin_sqrt(dot3(u, u));
-  u = V3{u.x / un, u.y / un, u.z / un};
-  const V3 w = cross3(b, u);
-  const double s = charge > 0.0 ? 1.0 : (charge < 0.0 ? -1.0 : 0.0);
-  const double cpa = cos(pa), spa = sin(pa);
-  const V3 vel{v * (cpa * b.x - s * spa * w.x), v * (cpa * b.y - s * spa * w.y),
-               v * (cpa * b.z - s * spa * w.z)};
-  const double gamma = 1.0 / __builtin_sqrt(1.0 - dot3(vel, vel) / C_LIGHT_SQ);
-  const double mg = mass * gamma;
-  y[0] = y[0] + rc * u.x;
-  y[1] = y[1] + rc * u.y;
-  y[2] = y[2] + rc * u.z;
-  y[3] = mg * vel.x;
-  y[4] = mg * vel.y;
-  y[5] = mg * vel.z;
-}
 
 // ---- the persistent tracer kernel ---------------------------------------------------
 template <int M, int KIND>
@@ -356,6 +164,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
     // ---- refill: take the next particle ---------------------------------------
     if (!active && !exhausted) {
       pi = (int64_t)atomicAdd(&A.work[0], 1ULL);
+      if (pi < n && A.order) pi = A.order[pi];
       if (pi >= n) {
         exhausted = true;
       } else {
@@ -714,6 +523,7 @@ struct rapt_batch {
   int64_t* d_naccpt = nullptr;
   int64_t* d_rowoff = nullptr;
   int64_t* d_rowcap = nullptr;
+  int64_t* d_order = nullptr;  // work order: particle indices by output step
   double* d_rows = nullptr;
   int32_t width = 0;
   int64_t total_rows = 0;
@@ -755,6 +565,8 @@ static CtlParams ctl(double safety, double dfactor, double ifactor, double beta,
   }
   return p;
 }
+
+void rapt_launch_gc4(hipStream_t st, int model, const KArgs& A, int block, int64_t groups);
 
 template <int M, int K>
 static void launch_trace(rapt_ctx* c, const KArgs& A, int block, int64_t lanes) {
@@ -879,6 +691,7 @@ int rapt_batch_create(rapt_ctx* c, int64_t n, rapt_batch** out) {
             hipMalloc(&b->d_naccpt, sizeof(int64_t) * n) == hipSuccess &&
             hipMalloc(&b->d_rowoff, sizeof(int64_t) * n) == hipSuccess &&
             hipMalloc(&b->d_rowcap, sizeof(int64_t) * n) == hipSuccess &&
+            hipMalloc(&b->d_order, sizeof(int64_t) * n) == hipSuccess &&
             hipMalloc(&b->d_work, sizeof(unsigned long long) * 8) == hipSuccess;
   if (!ok) {
     rapt_batch_destroy(b);
@@ -901,6 +714,7 @@ int rapt_batch_destroy(rapt_batch* b) {
   (void)hipFree(b->d_naccpt);
   (void)hipFree(b->d_rowoff);
   (void)hipFree(b->d_rowcap);
+  (void)hipFree(b->d_order);
   (void)hipFree(b->d_rows);
   (void)hipFree(b->d_work);
   delete b;
@@ -919,6 +733,21 @@ int rapt_batch_upload(rapt_batch* b, const double* state, const int32_t* flags) 
                            hipMemcpyDeviceToDevice, c->stream));
   HIP_OK(c, hipMemcpyAsync(b->d_flags0, b->d_flags, sizeof(int32_t) * b->n,
                            hipMemcpyDeviceToDevice, c->stream));
+  // Work order: shortest output step first.  For guiding centres the output
+  // step is a fixed fraction of the bounce period and the step count scales
+  // with delta / bounce period, so this starts the longest traces first and
+  // puts traces of similar length into the same wavefront.  Entries without
+  // an output step (0: computed in the kernel) keep their input order after
+  // the others.
+  std::vector<int64_t> order((size_t)b->n);
+  const double* dts = state + (size_t)RAPT_S_DT * b->n;
+  for (int64_t i = 0; i < b->n; ++i) order[(size_t)i] = i;
+  std::stable_sort(order.begin(), order.end(), [dts](int64_t a, int64_t q) {
+    const double ka = dts[a] > 0.0 ? dts[a] : HUGE_VAL, kq = dts[q] > 0.0 ? dts[q] : HUGE_VAL;
+    return ka < kq;
+  });
+  HIP_OK(c, hipMemcpyAsync(b->d_order, order.data(), sizeof(int64_t) * b->n,
+                           hipMemcpyHostToDevice, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
   return RAPT_OK;
 }
@@ -985,6 +814,10 @@ int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) 
     return set_err(c, RAPT_E_ARG, "PARTICLE mode steps with DOP853 or RKF45");
   if (d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOP853)
     return set_err(c, RAPT_E_ARG, "GC mode steps with DOPRI5 or RKF45");
+  if (d->lanes_per_particle != 0 && d->lanes_per_particle != 1 && d->lanes_per_particle != 4)
+    return set_err(c, RAPT_E_ARG, "lanes_per_particle must be 0, 1 or 4");
+  if (d->work_order != 0 && d->work_order != 1)
+    return set_err(c, RAPT_E_ARG, "work_order must be 0 or 1");
   if (d->precision != RAPT_PREC_FP64)
     return set_err(c, RAPT_E_ARG, "precision %d is not built", d->precision);
   if (d->mode == RAPT_MODE_ADAPTIVE && !(d->gctimestep > 0.0))
@@ -1038,15 +871,24 @@ int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) 
   A.row_cap = b->d_rowcap;
   A.rows = b->d_rows;
   A.work = b->d_work;
+  A.order = d->work_order == 0 ? b->d_order : nullptr;
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
   if (lanes > b->n) lanes = b->n;
   if (lanes < 1) lanes = 1;
+  const bool quad = d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOPRI5 &&
+                    d->lanes_per_particle != 1;
   HIP_OK(c, hipMemsetAsync(b->d_work, 0, sizeof(unsigned long long) * 8, c->stream));
   HIP_OK(c, hipEventRecord(c->ev0, c->stream));
-  switch (d->mode) {
-    case RAPT_MODE_PARTICLE: launch_model<KIND_PARTICLE>(c, A, block, lanes); break;
-    case RAPT_MODE_GC: launch_model<KIND_GC>(c, A, block, lanes); break;
-    default: launch_model<KIND_ADAPTIVE>(c, A, block, lanes); break;
+  if (quad) {
+    int64_t groups = d->lanes > 0 ? (d->lanes + 3) / 4 : b->n;
+    if (groups > b->n) groups = b->n;
+    rapt_launch_gc4(c->stream, c->field.model, A, block, groups);
+  } else {
+    switch (d->mode) {
+      case RAPT_MODE_PARTICLE: launch_model<KIND_PARTICLE>(c, A, block, lanes); break;
+      case RAPT_MODE_GC: launch_model<KIND_GC>(c, A, block, lanes); break;
+      default: launch_model<KIND_ADAPTIVE>(c, A, block, lanes); break;
+    }
   }
   HIP_OK(c, hipGetLastError());
   HIP_OK(c, hipEventRecord(c->ev1, c->stream));

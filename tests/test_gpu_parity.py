@@ -261,6 +261,81 @@ def test_resume_after_row_capacity_is_exact():
         np.testing.assert_array_equal(a, b)
 
 
+def _gc_batch(field, n, seed_pa=None):
+    wl = W.doubledipole_electrons_gc(n=n)
+    g = 1 / np.sqrt(1 - (wl["v"] / K.c) ** 2)
+    pa = np.radians(wl["pa"])
+    ppar = g * K.m_el * wl["v"] * np.cos(pa)
+    B = N.context().field_eval(field.descriptor(), "magB", np.c_[np.zeros(n), wl["pos"]])[:, 0]
+    mu = g ** 2 * K.m_el * (wl["v"] * np.sin(pa)) ** 2 / (2 * B)
+    return np.c_[np.zeros(n), wl["pos"], ppar], mu, wl["v"]
+
+
+@pytest.mark.parametrize("name,field,eom", [
+    ("tcb", lambda: F.DoubleDipole(), "TaoChanBrizardEOM"),
+    ("bc", lambda: F.DoubleDipole(), "BrizardChanEOM"),
+    ("nt", lambda: F.DoubleDipole(), "NorthropTellerEOM"),
+    ("vardipole", lambda: F.VarEarthDipole(amp=0.2, period=7.0), "TaoChanBrizardEOM"),
+    ("crossedEB", lambda: F.UniformCrossedEB(Ey=1e-3, Bz=1e-7), "TaoChanBrizardEOM"),
+    ("dipunif", lambda: F.DipolePlusUniform(), "TaoChanBrizardEOM"),
+])
+def test_gc_quad_lanes_match_single_lane(name, field, eom):
+    """Four lanes per guiding centre (the default) against one lane per
+    guiding centre: bit-identical rows, step counts and statuses.  The library
+    is built without FMA contraction and both kernels form every stencil
+    point, sum and norm in the same order, so any difference is a defect."""
+    f = field()
+    n = 40
+    last, mu, v = _gc_batch(f, n)
+    prm = P(solvertolerances=(1e-8, 1e-8))
+    a = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, eom=eom,
+                          lanes_per_particle=1)
+    b = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, eom=eom,
+                          lanes_per_particle=4)
+    np.testing.assert_array_equal(a.status, b.status)
+    np.testing.assert_array_equal(a.naccpt, b.naccpt)
+    for x, y in zip(a.rows, b.rows):
+        np.testing.assert_array_equal(y, x)
+    np.testing.assert_array_equal(b.tcur, a.tcur)
+    np.testing.assert_array_equal(b.state, a.state)
+
+
+def test_gc_quad_refill_and_adiabaticity_check():
+    """Quad kernel with fewer resident quads than guiding centres (work
+    refill) and the adiabaticity check on: bit-identical rows and statuses to
+    the one-lane kernel."""
+    f = F.DoubleDipole()
+    n = 150
+    last, mu, v = _gc_batch(f, n)
+    prm = P(solvertolerances=(1e-8, 1e-8), epss=0.02)
+    a = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, check=True,
+                          lanes_per_particle=1)
+    b = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, check=True,
+                          lanes_per_particle=4, lanes=64)
+    np.testing.assert_array_equal(a.status, b.status)
+    for x, y in zip(a.rows, b.rows):
+        np.testing.assert_array_equal(y, x)
+
+
+@pytest.mark.parametrize("lpp", [1, 4])
+def test_work_order_does_not_change_results(lpp):
+    """The library's work order (shortest output step first, the default)
+    and input order give bit-identical per-particle results."""
+    f = F.DoubleDipole()
+    n = 96
+    last, mu, v = _gc_batch(f, n)
+    dt = np.random.default_rng(3).uniform(0.02, 0.2, n)
+    prm = P(solvertolerances=(1e-8, 1e-8))
+    a = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, dt, 0.6, params=prm,
+                          lanes_per_particle=lpp, lanes=32, work_order=0)
+    b = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, dt, 0.6, params=prm,
+                          lanes_per_particle=lpp, lanes=32, work_order=1)
+    np.testing.assert_array_equal(a.status, b.status)
+    np.testing.assert_array_equal(a.state, b.state)
+    for x, y in zip(a.rows, b.rows):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_work_refill_is_exact():
     """Fewer resident lanes than particles (work refill) gives the same rows."""
     wl = W.dipole_protons_1mev(n=200)
