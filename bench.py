@@ -10,6 +10,12 @@ between steps, so inputs stay resident in HBM.  A particle-step is one
 accepted integrator step of one particle.
 
 Prints one JSON line.  With --gpus > 1 it exits: this build measures one GPU.
+
+Other workloads (--workload, one JSON line each; not the default line):
+  particle_dipole   1024 protons, 1 MeV, EarthDipole, Particle.advance(10), DOP853
+  adaptive_sheet    262,144 protons, current sheet + uniform Ey, Adaptive.advance(120)
+  mixed_precision   1,048,576 protons and electrons, EarthDipole - 10 nT z, advance(30):
+                    FP64 against FP32-field, errors per species
 """
 import argparse
 import json
@@ -81,13 +87,261 @@ def cpu_baseline(w, sample, threads):
     el = time.perf_counter() - t0
     return cnt.naccpt, el
 
+def timed_run(field, mode, S, flags, cap, desc, steps, warmup, width, rows=False):
+    """Upload once, then warmup + timed runs of the same batch (state reset on
+    the device between runs).  Returns (totals, seconds, last download)."""
+    import ctypes
+    from rapt_amd import _native as N
+    ctx = N.context(0)
+    ctx.set_field(field.descriptor())
+    b = N.Batch(ctx, S.shape[1])
+    b.upload(S, flags)
+    b.set_rows(width, cap)
+    for i in range(warmup):
+        b.reset()
+        c = b.run(desc)
+        log("warmup %d: %.1f ms" % (i, c.kernel_ms))
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+    tot = dict(naccpt=0, nrejct=0, nstep=0, nfcn=0, nswitch=0, kernel_ms=0.0)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        b.reset()
+        c = b.run(desc)
+        for k in ("naccpt", "nrejct", "nstep", "nfcn", "nswitch"):
+            tot[k] += getattr(c, k)
+        tot["kernel_ms"] += c.kernel_ms
+        log("step %d: %.1f ms, %d accepted steps" % (i, c.kernel_ms, c.naccpt))
+    hip.hipDeviceSynchronize()
+    el = time.perf_counter() - t0
+    out = b.download(rows=rows)
+    out["row_off"] = np.asarray(b.row_off, np.int64).copy()
+    b.close()
+    return tot, el, out
+
+
+def flat_rows(out, idx):
+    """Rows of particles idx from a flat download: (rows, owner, group starts)."""
+    nr = out["nrows"][idx]
+    owner = np.repeat(idx, nr)
+    first = np.r_[0, np.cumsum(nr)[:-1]]
+    pos = np.arange(int(nr.sum())) - np.repeat(first, nr)
+    g = out["row_off"][owner] + pos
+    return out["rows"][g], owner, first
+
+
+def base_line(metric, value, unit, args, el, dtype, config):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic (seeded)", "config": config}
+
+
+def run_particle_dipole(args):
+    """1024 protons at 1 MeV in EarthDipole, Particle.advance(10), DOP853."""
+    import rapt_amd as rapt
+    from rapt_amd import _native as N, batch as BT, workloads as W
+    n = args.n or 1024
+    wl = W.dipole_protons_1mev(n=n)
+    f = rapt.fields.EarthDipole()
+    g = 1 / np.sqrt(1 - np.sum(wl["vel"] ** 2, axis=1) / rapt.c ** 2)
+    last7 = np.zeros((n, 7))
+    last7[:, 1:4] = wl["pos"]
+    last7[:, 4:7] = (rapt.m_pr * g)[:, None] * wl["vel"]
+    S = BT.particle_state(last7, rapt.m_pr, rapt.e)
+    dt = BT.particle_dt(f, last7, rapt.m_pr, rapt.e, 20)
+    cap = BT._rows_needed(10.0, dt)
+    desc = N.make_desc(N.MODE_PARTICLE, dict(rapt.params), 10.0, lanes=args.lanes)
+    tot, el, out = timed_run(f, N.MODE_PARTICLE, S, np.zeros(n, np.int32), cap, desc,
+                             args.steps, args.warmup, 7)
+    value = tot["naccpt"] / el
+    res = base_line("FP64 particle-steps/s (1 GPU)", value, "particle-steps/s", args, el, "f64",
+                    {"workload": "EarthDipole Particle batch: %d protons, 1 MeV, r U[4,7] Re, "
+                                 "pitch U[10,90] deg; DOP853 default tolerances, advance(10), "
+                                 "dt=T_c/20" % n, "particles": n})
+    res["accepted_steps_per_step"] = tot["naccpt"] / args.steps
+    res["kernel_ms_avg"] = tot["kernel_ms"] / args.steps
+    res["particles_done"] = int(np.sum(out["status"] == 1))
+    print(json.dumps(res), flush=True)
+
+
+def run_adaptive_sheet(args):
+    """262,144 protons, B = (20 nT tanh(z/0.5 Re), 0, 1 nT), E = 0.3 mV/m y,
+    Adaptive.advance(120) with GCtimestep 0.5 s."""
+    import rapt_amd as rapt
+    from rapt_amd import _native as N, batch as BT, workloads as W
+    n = args.n or 262144
+    wl = W.current_sheet_protons(n=n)
+    f = rapt.fields.CurrentSheet()
+    prm = dict(rapt.params, GCtimestep=0.5)
+    m = rapt.m_pr
+    g = 1 / np.sqrt(1 - np.sum(wl["vel"] ** 2, axis=1) / rapt.c ** 2)
+    last7 = np.zeros((n, 7))
+    last7[:, 1:4] = wl["pos"]
+    last7[:, 4:7] = (m * g)[:, None] * wl["vel"]
+    S = BT.particle_state(last7, m, rapt.e)
+    flags = np.full(n, N.FLAG_AINIT, np.int32)
+    # rows: Particle segments may reach the strongest field of the sheet
+    # (sqrt(B0^2 + Bn^2)), and every switch adds a segment start row
+    bmax = np.hypot(f.B0, f.Bn)
+    dt_min = 2 * np.pi * m / (rapt.e * bmax) / prm["cyclotronresolution"]
+    cap = np.full(n, int(np.ceil(120.0 / min(dt_min, 0.5))) + 200, np.int64)
+    desc = N.make_desc(N.MODE_ADAPTIVE, prm, 120.0, lanes=args.lanes)
+    tot, el, out = timed_run(f, N.MODE_ADAPTIVE, S, flags, cap, desc, args.steps, args.warmup,
+                             8)
+    value = tot["naccpt"] / el
+    res = base_line("FP64 particle-steps/s (1 GPU)", value, "particle-steps/s", args, el, "f64",
+                    {"workload": "current sheet Adaptive batch: %d protons, log10 K[eV] U[3,5], "
+                                 "isotropic; B0 20 nT, lam 0.5 Re, Bn 1 nT, Ey 0.3 mV/m; "
+                                 "advance(120), GCtimestep 0.5 s" % n, "particles": n})
+    res["accepted_steps_per_step"] = tot["naccpt"] / args.steps
+    res["switches_per_step"] = tot["nswitch"] / args.steps
+    res["kernel_ms_avg"] = tot["kernel_ms"] / args.steps
+    st = out["status"]
+    res["status_counts"] = {str(k): int(np.sum(st == k)) for k in np.unique(st)}
+    print(json.dumps(res), flush=True)
+
+
+def _pct(a):
+    a = np.asarray(a, float)
+    if a.size == 0:
+        return None
+    return {"median": float(np.median(a)), "p99": float(np.percentile(a, 99)),
+            "max": float(np.max(a))}
+
+
+def run_mixed_precision(args):
+    """Mixed precision: 2^20 particles, half protons, half
+    electrons, EarthDipole + (0, 0, -10 nT), advance(30).  The whole
+    population runs as guiding centres (output every 1 s) in FP64 and in
+    FP32-field; a Particle-mode sub-batch (256 of each species; protons 30 s,
+    electrons 0.02 s) gives the first-invariant comparison.  FP32-field runs
+    use the I-controller (beta = 0); the comparisons are against FP64 with
+    the same controller, and the FP64 controller change is reported apart."""
+    import rapt_amd as rapt
+    from rapt_amd import _native as N, batch as BT, workloads as W
+    n = args.n or 1048576
+    wl = W.mixed_dipole_uniform(n=n)
+    f = rapt.fields.DipolePlusUniform(Bu=(0.0, 0.0, wl["dBz"]))
+    ctx = N.context(0)
+    m, q, v, pa = wl["mass"], wl["charge"], wl["v"], np.radians(wl["pa"])
+    g = 1 / np.sqrt(1 - (v / rapt.c) ** 2)
+    B0 = ctx.field_eval(f.descriptor(), "magB", np.c_[np.zeros(n), wl["pos"]])[:, 0]
+    ppar = g * m * v * np.cos(pa)
+    mu = g ** 2 * m * (v * np.sin(pa)) ** 2 / (2 * B0)
+    last5 = np.c_[np.zeros(n), wl["pos"], ppar]
+    dt_out = 1.0
+    S = BT.gc_state(last5, m, q, mu, v, dt_out)
+    cap = np.full(n, 33, np.int64)
+    prm = dict(rapt.params)
+    res_gc = {}
+    outs = {}
+    # fp32_field runs the I-controller (beta = 0); fp64_beta0 isolates the
+    # precision effect, fp64 (dopri5 default beta) is the reference setting
+    for name, kw in (("fp64", {}), ("fp64_beta0", {"beta": -1.0}),
+                     ("fp32_field", {"precision": "fp32_field"})):
+        desc = N.make_desc(N.MODE_GC, prm, 30.0, lanes_per_particle=1, **kw)
+        tot, el, out = timed_run(f, N.MODE_GC, S, np.zeros(n, np.int32), cap, desc,
+                                 args.steps, args.warmup, 5, rows=True)
+        outs[name] = out
+        res_gc[name] = {"particle_steps_per_s": tot["naccpt"] / el,
+                        "kernel_ms_avg": tot["kernel_ms"] / args.steps,
+                        "accepted_steps": tot["naccpt"] / args.steps,
+                        "done": int(np.sum(out["status"] == 1))}
+        log("GC %s: %.3g particle-steps/s" % (name, tot["naccpt"] / el))
+    # FP64 on the four-lane kernel (the production FP64 path) for the speed line
+    desc = N.make_desc(N.MODE_GC, prm, 30.0)
+    tot, el, _ = timed_run(f, N.MODE_GC, S, np.zeros(n, np.int32), cap, desc, args.steps,
+                           args.warmup, 5)
+    res_gc["fp64_quad"] = {"particle_steps_per_s": tot["naccpt"] / el,
+                           "kernel_ms_avg": tot["kernel_ms"] / args.steps}
+    # per-species errors at the output rows
+    a, b, a0 = outs["fp64_beta0"], outs["fp32_field"], outs["fp64"]
+    errs = {}
+    for sp, name in ((0, "proton"), (1, "electron")):
+        idx = np.nonzero((wl["species"] == sp) & (a["status"] == 1) & (b["status"] == 1) &
+                         (a0["status"] == 1) & (a["nrows"] == b["nrows"]) &
+                         (a["nrows"] == a0["nrows"]) & (a["nrows"] > 0))[0]
+        ra, owner, first = flat_rows(a, idx)
+        rb, _, _ = flat_rows(b, idx)
+        r0, _, _ = flat_rows(a0, idx)
+        d = np.linalg.norm(ra[:, 1:4] - rb[:, 1:4], axis=1) / np.linalg.norm(ra[:, 1:4], axis=1)
+        dc = np.linalg.norm(ra[:, 1:4] - r0[:, 1:4], axis=1) / np.linalg.norm(r0[:, 1:4], axis=1)
+        mc = m[owner] * rapt.c
+        Ba = ctx.field_eval(f.descriptor(), "magB", ra[:, :4])[:, 0]
+        Bb = ctx.field_eval(f.descriptor(), "magB", rb[:, :4])[:, 0]
+        ga = np.sqrt(1 + 2 * mu[owner] * Ba / (mc * rapt.c) + (ra[:, 4] / mc) ** 2)
+        gb = np.sqrt(1 + 2 * mu[owner] * Bb / (mc * rapt.c) + (rb[:, 4] / mc) ** 2)
+        Ka, Kb = (ga - 1) * mc * rapt.c, (gb - 1) * mc * rapt.c
+        K0 = (g[owner] - 1) * mc * rapt.c
+        red = lambda x: np.maximum.reduceat(x, first)
+        errs[name] = {"particles": int(len(idx)),
+                      "gc_position_rel_err_fp32_vs_fp64": _pct(red(d)),
+                      "gc_position_rel_err_beta0_vs_default_fp64": _pct(red(dc)),
+                      "gc_energy_drift_fp64": _pct(red(np.abs(Ka - K0) / K0)),
+                      "gc_energy_drift_fp32": _pct(red(np.abs(Kb - K0) / K0)),
+                      "gc_energy_rel_err_fp32_vs_fp64": _pct(red(np.abs(Kb - Ka) / Ka))}
+    # Particle-mode sub-batch: first invariant mu = p_perp^2 / (2 m B) per row
+    for sp, name, delta in ((0, "proton", 30.0), (1, "electron", 0.02)):
+        sel = np.nonzero(wl["species"] == sp)[0][:256]
+        k = sel.size
+        last7 = np.zeros((k, 7))
+        last7[:, 1:4] = wl["pos"][sel]
+        last7[:, 4:7] = (m[sel] * g[sel])[:, None] * wl["vel"][sel]
+        mus = {}
+        rows = {}
+        for prec in ("fp64", "fp32_field"):
+            r = BT.advance_particles(f, last7, m[sel], q[sel], delta, params=prm,
+                                     precision=prec, beta=0.0)
+            rows[prec] = r
+            mm = []
+            for i in range(k):
+                rr = np.vstack([last7[i:i + 1], r.rows[i]])
+                Bv = ctx.field_eval(f.descriptor(), "B", rr[:, :4])
+                Bm = np.linalg.norm(Bv, axis=1)
+                pb = np.sum(rr[:, 4:7] * Bv, axis=1) / Bm
+                pperp2 = np.sum(rr[:, 4:7] ** 2, axis=1) - pb ** 2
+                mm.append(pperp2 / (2 * m[sel][i] * Bm))
+            mus[prec] = mm
+        drift64 = [np.max(np.abs(x - x[0])) / x[0] for x in mus["fp64"]]
+        drift32 = [np.max(np.abs(x - x[0])) / x[0] for x in mus["fp32_field"]]
+        mu_err = [np.max(np.abs(x[:min(len(x), len(y))] - y[:min(len(x), len(y))]) /
+                         y[:min(len(x), len(y))])
+                  for x, y in zip(mus["fp32_field"], mus["fp64"])]
+        pe = []
+        for i in range(k):
+            x64, x32 = rows["fp64"].rows[i], rows["fp32_field"].rows[i]
+            kk = min(len(x64), len(x32))
+            pe.append(np.max(np.linalg.norm(x64[:kk, 1:4] - x32[:kk, 1:4], axis=1) /
+                             np.linalg.norm(x64[:kk, 1:4], axis=1)))
+        errs[name].update({"particle_subbatch": {"particles": k, "delta_s": delta,
+                                                 "mu_drift_fp64": _pct(drift64),
+                                                 "mu_drift_fp32": _pct(drift32),
+                                                 "mu_rel_err_fp32_vs_fp64": _pct(mu_err),
+                                                 "position_rel_err_fp32_vs_fp64": _pct(pe)}})
+    value = res_gc["fp32_field"]["particle_steps_per_s"]
+    res = base_line("FP32-field guiding-centre particle-steps/s (1 GPU)", value,
+                    "particle-steps/s", args, args.steps * res_gc["fp32_field"]["kernel_ms_avg"]
+                    / 1e3, "f32 field / f64 state",
+                    {"workload": "EarthDipole + (0,0,-10 nT) GuidingCenter batch: %d particles "
+                                 "(half protons, half electrons), log10 K[eV] U[5,log10 5e6], "
+                                 "L U[3,7], pitch U[10,90] deg; DOPRI5 default tolerances, "
+                                 "advance(30), output every 1 s" % n, "particles": n})
+    res["gc_runs"] = res_gc
+    res["errors_per_species"] = errs
+    print(json.dumps(res), flush=True)
+
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="gc_doubledipole",
+                    choices=("gc_doubledipole", "particle_dipole", "adaptive_sheet",
+                             "mixed_precision"))
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--lpp", type=int, default=4, choices=(1, 4),
                     help="lanes per guiding centre (4: one lane quad per particle)")
@@ -116,6 +370,11 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe}))
         return
 
+    if args.workload != "gc_doubledipole":
+        {"particle_dipole": run_particle_dipole, "adaptive_sheet": run_adaptive_sheet,
+         "mixed_precision": run_mixed_precision}[args.workload](args)
+        return
+    args.n = args.n or 65536
     from rapt_amd import _native as N
     w = build_workload(args.n)
     n = args.n

@@ -62,6 +62,15 @@ enum rapt_mode {
 };
 enum rapt_eom { RAPT_EOM_TCB = 0, RAPT_EOM_BC = 1, RAPT_EOM_NT = 2 };
 enum rapt_stepper { RAPT_STEP_DOP853 = 0, RAPT_STEP_DOPRI5 = 1, RAPT_STEP_RKF45 = 2 };
+/* FP32_FIELD: field evaluations and the finite-difference drift terms in
+ * float (gradient step max(gradientstepsize, |r|/1024)), state, stage sums,
+ * error norm and step control in double.  The error norm scales each
+ * component by the largest component of its 3-vector (position, momentum)
+ * instead of componentwise, so the float rounding of the forces stays below
+ * tolerance where a component passes through zero, and the step controller
+ * runs with beta = 0 (I-controller): a PI term settles on an error level
+ * below the float rounding floor and shrinks the step without bound.  Not a
+ * reference mode. */
 enum rapt_precision { RAPT_PREC_FP64 = 0, RAPT_PREC_FP32_FIELD = 1 };
 
 /* Per-launch settings: rapt.params (rapt/__init__.py:21-34) plus the scipy

@@ -18,6 +18,7 @@ LIB_PATH = os.environ.get("RAPT_HIP_LIB") or os.path.join(_HERE, "lib", "librapt
 MODE_PARTICLE, MODE_GC, MODE_ADAPTIVE = 0, 1, 2
 EOM = {"TaoChanBrizardEOM": 0, "BrizardChanEOM": 1, "NorthropTellerEOM": 2}
 STEP_DOP853, STEP_DOPRI5, STEP_RKF45 = 0, 1, 2
+PREC_FP64, PREC_FP32_FIELD = 0, 1
 NSLOT = 16
 S_T, S_Y0, S_MASS, S_CHARGE, S_MU, S_V, S_DT, S_H, S_TEND, S_GM, S_TCUR = \
     0, 1, 7, 8, 9, 10, 11, 12, 13, 14, 15
@@ -254,7 +255,7 @@ def make_desc(mode, params, delta, eom="TaoChanBrizardEOM", stepper=None, check=
     if stepper is None:
         stepper = STEP_DOPRI5 if mode == MODE_GC else STEP_DOP853
     d.stepper = stepper
-    d.precision = 0
+    d.precision = {"fp64": PREC_FP64, "fp32_field": PREC_FP32_FIELD}[kw.get("precision", "fp64")]
     d.nsteps = int(kw.get("nsteps", 500))
     d.enforce_equatorial = 1 if params.get("enforce equatorial", False) else 0
     d.check_adiabaticity = 1 if check else 0

@@ -44,9 +44,12 @@ struct FieldC {
   double p[16];
 };
 
-struct V3 {
-  double x, y, z;
+template <class T>
+struct V3T {
+  T x, y, z;
 };
+using V3 = V3T<double>;
+using V3f = V3T<float>;
 
 // 1/sqrt(x) and 1/x from the hardware estimates (v_rsq_f64, v_rcp_f64) with
 // two Newton steps each: full double precision to within an ulp or two, in 9
@@ -66,76 +69,115 @@ __device__ __forceinline__ double rcp_nr(double x) {
   e = __builtin_fma(-x, r, 1.0);
   return __builtin_fma(r, e, r);
 }
+// single precision: v_rsq_f32 / v_rcp_f32 are accurate to about 1 ulp
+__device__ __forceinline__ float rsqrt_nr(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ float rcp_nr(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double absT(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float absT(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double sqrtT(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float sqrtT(float a) { return __builtin_sqrtf(a); }
+
 // |r|^-5 from r^2
-__device__ __forceinline__ double inv_r5(double r2) {
-  const double rs = rsqrt_nr(r2);
-  const double i2 = rs * rs;
+template <class T>
+__device__ __forceinline__ T inv_r5(T r2) {
+  const T rs = rsqrt_nr(r2);
+  const T i2 = rs * rs;
   return i2 * i2 * rs;
 }
-
-__device__ __forceinline__ double dot3(const V3& a, const V3& b) {
-  // same association as numpy's 3-element dot: fma(a2,b2, fma(a1,b1, a0*b0))
-  return __builtin_fma(a.z, b.z, __builtin_fma(a.y, b.y, a.x * b.x));
-}
-__device__ __forceinline__ V3 cross3(const V3& a, const V3& b) {
-  return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-
-// ---- B(t, r) ---------------------------------------------------------------
-template <int M>
-__device__ __forceinline__ V3 fieldB(const FieldC& F, double t, double x, double y, double z) {
-  if constexpr (M == F_EARTH_DIPOLE || M == F_DIPOLE_UNIFORM) {
-    const double r2 = x * x + y * y + z * z;
-    const double s = F.p[0] * inv_r5(r2);
-    V3 B{s * (x * z), s * (y * z), s * (z * z - r2 * (1.0 / 3.0))};
-    if constexpr (M == F_DIPOLE_UNIFORM) {
-      B.x += F.p[1];
-      B.y += F.p[2];
-      B.z += F.p[3];
-    }
-    return B;
-  } else if constexpr (M == F_DOUBLE_DIPOLE) {
-    const double yz2 = y * y + z * z;
-    const double r2a = x * x + yz2;
-    const double ia = inv_r5(r2a);
-    const double xb = x - F.p[1];
-    const double r2b = xb * xb + yz2;
-    const double ib = F.p[2] * inv_r5(r2b);
-    const double c = F.p[0];
-    const double zz2 = 2.0 * z * z;
-    return V3{c * ((3.0 * x * z) * ia + (3.0 * xb * z) * ib),
-              c * ((3.0 * y * z) * ia + (3.0 * y * z) * ib),
-              c * ((zz2 - x * x - y * y) * ia + (zz2 - xb * xb - y * y) * ib)};
-  } else if constexpr (M == F_UNIFORM_BZ || M == F_UNIFORM_CROSSED_EB) {
-    return V3{0.0, 0.0, F.p[0]};
-  } else if constexpr (M == F_VAR_EARTH_DIPOLE) {
-    const double s = F.p[0] * (1.0 + F.p[1] * sin(TWO_PI * t / F.p[2]));
-    const double r2 = x * x + y * y + z * z;
-    const double iv = s * inv_r5(r2);
-    return V3{(3.0 * x * z) * iv, (3.0 * y * z) * iv, (2.0 * z * z - x * x - y * y) * iv};
-  } else if constexpr (M == F_PARABOLIC) {
-    double bx;
-    if (__builtin_fabs(z) <= 1.0) {
-      bx = F.p[0] * z / F.p[2];
-    } else {
-      bx = (z > 0.0 ? 1.0 : (z < 0.0 ? -1.0 : 0.0)) * F.p[3];
-    }
-    return V3{bx, 0.0, F.p[1]};
-  } else if constexpr (M == F_CURRENT_SHEET) {
-    return V3{F.p[0] * tanh(z / F.p[1]), 0.0, F.p[2]};
+// c |r|^-5.  In double, c * inv_r5(r2).  In float |r|^-5 underflows (4e7 m
+// gives 1e-38), so the coefficient is folded in first: ((c rs) rs^2) rs^2.
+template <class T>
+__device__ __forceinline__ T coef_r5(T c, T r2) {
+  if constexpr (sizeof(T) == sizeof(double)) {
+    return c * inv_r5(r2);
   } else {
-    return V3{0.0, 0.0, 0.0};
+    const T rs = rsqrt_nr(r2);
+    const T i2 = rs * rs;
+    return ((c * rs) * i2) * i2;
   }
 }
 
-template <int M>
-__device__ __forceinline__ V3 fieldE(const FieldC& F) {
-  if constexpr (M == F_UNIFORM_CROSSED_EB) {
-    return V3{0.0, F.p[1], 0.0};
+template <class T>
+__device__ __forceinline__ T dot3(const V3T<T>& a, const V3T<T>& b) {
+  // same association as numpy's 3-element dot: fma(a2,b2, fma(a1,b1, a0*b0))
+  return fmaT(a.z, b.z, fmaT(a.y, b.y, a.x * b.x));
+}
+template <class T>
+__device__ __forceinline__ V3T<T> cross3(const V3T<T>& a, const V3T<T>& b) {
+  return V3T<T>{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// ---- B(t, r) ---------------------------------------------------------------
+// T = double is the tracer's arithmetic; T = float serves the FP32-field
+// precision mode (constants rounded once to float).
+template <int M, class T = double>
+__device__ __forceinline__ V3T<T> fieldB(const FieldC& F, T t, T x, T y, T z) {
+  using V = V3T<T>;
+  if constexpr (M == F_EARTH_DIPOLE || M == F_DIPOLE_UNIFORM) {
+    const T r2 = x * x + y * y + z * z;
+    const T s = coef_r5(T(F.p[0]), r2);
+    V B{s * (x * z), s * (y * z), s * (z * z - r2 * T(1.0 / 3.0))};
+    if constexpr (M == F_DIPOLE_UNIFORM) {
+      B.x += T(F.p[1]);
+      B.y += T(F.p[2]);
+      B.z += T(F.p[3]);
+    }
+    return B;
+  } else if constexpr (M == F_DOUBLE_DIPOLE) {
+    const T yz2 = y * y + z * z;
+    const T r2a = x * x + yz2;
+    const T xb = x - T(F.p[1]);
+    const T r2b = xb * xb + yz2;
+    const T zz2 = T(2.0) * z * z;
+    if constexpr (sizeof(T) == sizeof(double)) {
+      const T ia = inv_r5(r2a);
+      const T ib = T(F.p[2]) * inv_r5(r2b);
+      const T c = T(F.p[0]);
+      return V{c * ((T(3.0) * x * z) * ia + (T(3.0) * xb * z) * ib),
+               c * ((T(3.0) * y * z) * ia + (T(3.0) * y * z) * ib),
+               c * ((zz2 - x * x - y * y) * ia + (zz2 - xb * xb - y * y) * ib)};
+    } else {
+      const T ca = coef_r5(T(F.p[0]), r2a);
+      const T cb = coef_r5(T(F.p[0]) * T(F.p[2]), r2b);
+      return V{(T(3.0) * x * z) * ca + (T(3.0) * xb * z) * cb,
+               (T(3.0) * y * z) * ca + (T(3.0) * y * z) * cb,
+               (zz2 - x * x - y * y) * ca + (zz2 - xb * xb - y * y) * cb};
+    }
+  } else if constexpr (M == F_UNIFORM_BZ || M == F_UNIFORM_CROSSED_EB) {
+    return V{T(0.0), T(0.0), T(F.p[0])};
+  } else if constexpr (M == F_VAR_EARTH_DIPOLE) {
+    const T s = T(F.p[0]) * (T(1.0) + T(F.p[1]) * sin(T(TWO_PI) * t / T(F.p[2])));
+    const T r2 = x * x + y * y + z * z;
+    const T iv = coef_r5(s, r2);
+    return V{(T(3.0) * x * z) * iv, (T(3.0) * y * z) * iv,
+             (T(2.0) * z * z - x * x - y * y) * iv};
+  } else if constexpr (M == F_PARABOLIC) {
+    T bx;
+    if (absT(z) <= T(1.0)) {
+      bx = T(F.p[0]) * z / T(F.p[2]);
+    } else {
+      bx = (z > T(0.0) ? T(1.0) : (z < T(0.0) ? T(-1.0) : T(0.0))) * T(F.p[3]);
+    }
+    return V{bx, T(0.0), T(F.p[1])};
   } else if constexpr (M == F_CURRENT_SHEET) {
-    return V3{F.p[3], F.p[4], F.p[5]};
+    return V{T(F.p[0]) * tanh(z / T(F.p[1])), T(0.0), T(F.p[2])};
   } else {
-    return V3{0.0, 0.0, 0.0};
+    return V{T(0.0), T(0.0), T(0.0)};
+  }
+}
+
+template <int M, class T = double>
+__device__ __forceinline__ V3T<T> fieldE(const FieldC& F) {
+  if constexpr (M == F_UNIFORM_CROSSED_EB) {
+    return V3T<T>{T(0.0), T(F.p[1]), T(0.0)};
+  } else if constexpr (M == F_CURRENT_SHEET) {
+    return V3T<T>{T(F.p[3]), T(F.p[4]), T(F.p[5])};
+  } else {
+    return V3T<T>{T(0.0), T(0.0), T(0.0)};
   }
 }
 
@@ -286,20 +328,18 @@ __device__ __forceinline__ void rhs_lorentz(const FieldC& F, double charge, doub
 enum Eom : int { EOM_TCB = 0, EOM_BC = 1, EOM_NT = 2 };
 
 // Guiding-centre equations (rapt/GuidingCenter.py:329-395).  Y = (x, y, z, ppar).
-template <int M>
-__device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, double mass,
-                                       double mu, double v, bool eq, double t, const double* Y,
-                                       double* out) {
+// The algebra, given B, grad|B|, curl b and (non-static TCB only) db/dt at the
+// guiding centre; always FP64.
+__device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, bool is_static,
+                                           double charge, double mass, double mu, double v,
+                                           bool eq, const double* Y, const V3& B, const V3& gB,
+                                           const V3& cb, const V3& db, double* out) {
   const double ppar = Y[3];
-  const V3 B = fieldB<M>(F, t, Y[0], Y[1], Y[2]);
   const double bb = dot3(B, B);
   const double ibm = rsqrt_nr(bb);
   const double Bmag = bb * ibm;
   const V3 ub{B.x * ibm, B.y * ibm, B.z * ibm};
-  V3 gB, cb;
   if (eom == EOM_NT) {
-    // Northrop-Teller needs grad|B| only; the shared stencil costs the same
-    grad_curl<M>(F, t, Y[0], Y[1], Y[2], gB, cb);
     const double gamma = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
     const double gm = gamma * mass;
     const double coef = (gm * (v * v) + (ppar * ppar) / gm) / (2.0 * charge * (Bmag * Bmag));
@@ -310,7 +350,6 @@ __device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, 
     out[2] = coef * c.z + pg * ub.z;
     out[3] = -mu * dot3(ub, gB) / gamma;
   } else {
-    grad_curl<M>(F, t, Y[0], Y[1], Y[2], gB, cb);
     const double iq = rcp_nr(charge);
     const V3 Bs{B.x + ppar * cb.x * iq, B.y + ppar * cb.y * iq, B.z + ppar * cb.z * iq};
     const double Bspar = dot3(Bs, ub);
@@ -331,14 +370,12 @@ __device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, 
       const double igam = rcp_nr(gamma);
       const double mg = mu * igam;
       V3 Es;
-      if constexpr (has_E<M>()) {
-        const V3 E = fieldE<M>(F);
+      if (has_e) {
         Es = V3{E.x - mg * gB.x * iq, E.y - mg * gB.y * iq, E.z - mg * gB.z * iq};
       } else {
         Es = V3{-mg * gB.x * iq, -mg * gB.y * iq, -mg * gB.z * iq};
       }
-      if (!F.is_static) {
-        const V3 db = dbdt<M>(F, t, Y[0], Y[1], Y[2]);
+      if (!is_static) {
         Es.x -= ppar * db.x * iq;
         Es.y -= ppar * db.y * iq;
         Es.z -= ppar * db.z * iq;
@@ -354,6 +391,116 @@ __device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, 
   if (eq) {
     out[2] = 0.0;
     out[3] = 0.0;
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, double mass,
+                                       double mu, double v, bool eq, double t, const double* Y,
+                                       double* out) {
+  const V3 B = fieldB<M>(F, t, Y[0], Y[1], Y[2]);
+  V3 gB, cb, db{0.0, 0.0, 0.0};
+  grad_curl<M>(F, t, Y[0], Y[1], Y[2], gB, cb);
+  if (eom == EOM_TCB && !F.is_static) db = dbdt<M>(F, t, Y[0], Y[1], Y[2]);
+  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, eq, Y, B,
+             gB, cb, db, out);
+}
+
+// ---- FP32-field precision mode ----------------------------------------------------
+// Field evaluations and the finite-difference drift terms (grad|B|, curl b,
+// db/dt) in float; the state, stage sums, error norm and the guiding-centre
+// algebra stay in double (products like ppar^2 ~ 1e-44 underflow float).
+// A float stencil cannot resolve the FP64 gradient step (6.4 m at 4e7 m is
+// below float spacing), so it uses d = max(gradientstepsize, |r| 2^-10),
+// near the optimum of truncation (~(d/L)^2) against rounding (~eps_f32 L/d),
+// and divides by the distance the rounded float coordinates actually span.
+template <int M>
+__device__ __forceinline__ void gc_fields_f32(const FieldC& F, double t, double x, double y,
+                                              double z, V3& B0, V3& gB, V3& cb, V3& db,
+                                              bool need_db) {
+  const float tf = (float)t, xf = (float)x, yf = (float)y, zf = (float)z;
+  const V3f Bc = fieldB<M, float>(F, tf, xf, yf, zf);
+  B0 = V3{Bc.x, Bc.y, Bc.z};
+  const double d = fmax(F.gstep, __builtin_sqrt(x * x + y * y + z * z) * 0x1p-10);
+  const double c3[3] = {x, y, z};
+  float m[6];
+  V3f u[6];
+  double inv[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float cp = (float)(c3[k] + d), cm = (float)(c3[k] - d);
+    inv[k] = 1.0 / ((double)cp - (double)cm);
+#pragma unroll
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const float ck = sgn ? cm : cp;
+      const V3f B = fieldB<M, float>(F, tf, k == 0 ? ck : xf, k == 1 ? ck : yf, k == 2 ? ck : zf);
+      const float bb = dot3(B, B);
+      const float ib = rsqrt_nr(bb);
+      m[2 * k + sgn] = bb * ib;
+      u[2 * k + sgn] = V3f{B.x * ib, B.y * ib, B.z * ib};
+    }
+  }
+  gB = V3{(double)(m[0] - m[1]) * inv[0], (double)(m[2] - m[3]) * inv[1],
+          (double)(m[4] - m[5]) * inv[2]};
+  const double dby_dx = (double)(u[0].y - u[1].y) * inv[0];
+  const double dbz_dx = (double)(u[0].z - u[1].z) * inv[0];
+  const double dbx_dy = (double)(u[2].x - u[3].x) * inv[1];
+  const double dbz_dy = (double)(u[2].z - u[3].z) * inv[1];
+  const double dbx_dz = (double)(u[4].x - u[5].x) * inv[2];
+  const double dby_dz = (double)(u[4].y - u[5].y) * inv[2];
+  cb = V3{dbz_dy - dby_dz, dbx_dz - dbz_dx, dby_dx - dbx_dy};
+  db = V3{0.0, 0.0, 0.0};
+  if (need_db) {
+    const float tp = (float)(t + F.tstep), tm = (float)(t - F.tstep);
+    const double it = 1.0 / ((double)tp - (double)tm);
+    const V3f b1 = fieldB<M, float>(F, tm, xf, yf, zf);
+    const V3f b2 = fieldB<M, float>(F, tp, xf, yf, zf);
+    const float i1 = rsqrt_nr(dot3(b1, b1)), i2 = rsqrt_nr(dot3(b2, b2));
+    db = V3{(double)(b2.x * i2 - b1.x * i1) * it, (double)(b2.y * i2 - b1.y * i1) * it,
+            (double)(b2.z * i2 - b1.z * i1) * it};
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void rhs_gc_f32(const FieldC& F, int eom, double charge, double mass,
+                                           double mu, double v, bool eq, double t,
+                                           const double* Y, double* out) {
+  V3 B, gB, cb, db;
+  gc_fields_f32<M>(F, t, Y[0], Y[1], Y[2], B, gB, cb, db, eom == EOM_TCB && !F.is_static);
+  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, eq, Y, B,
+             gB, cb, db, out);
+}
+
+// Lorentz force with the field and the force in float; gamma*m in double.
+template <int M>
+__device__ __forceinline__ void rhs_lorentz_f32(const FieldC& F, double charge, double mass,
+                                                double& gm, bool eq, double t, const double* Y,
+                                                double* out) {
+  if (!F.is_static) {
+    const V3 pd{Y[3], Y[4], Y[5]};
+    gm = __builtin_sqrt(mass * mass + dot3(pd, pd) / C_LIGHT_SQ);
+  }
+  const float ig = (float)rcp_nr(gm);
+  const V3f p{(float)Y[3], (float)Y[4], (float)Y[5]};
+  out[0] = (double)(p.x * ig);
+  out[1] = (double)(p.y * ig);
+  out[2] = (double)(p.z * ig);
+  const V3f B = fieldB<M, float>(F, (float)t, (float)Y[0], (float)Y[1], (float)Y[2]);
+  const V3f c = cross3(p, B);
+  const float q = (float)charge;
+  if constexpr (has_E<M>()) {
+    const V3f E = fieldE<M, float>(F);
+    out[3] = (double)(q * (E.x + c.x * ig));
+    out[4] = (double)(q * (E.y + c.y * ig));
+    out[5] = (double)(q * (E.z + c.z * ig));
+  } else {
+    out[3] = (double)(q * (c.x * ig));
+    out[4] = (double)(q * (c.y * ig));
+    out[5] = (double)(q * (c.z * ig));
+  }
+  if (eq) {
+    out[2] = 0.0;
+    out[5] = 0.0;
   }
 }
 

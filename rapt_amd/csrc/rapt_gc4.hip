@@ -80,74 +80,21 @@ __device__ __forceinline__ double rhs_gc_quad(const FieldC& F, int eom, double c
   const V3 gB{qb<1>(e0), qb<2>(e0), qb<3>(e0)};
   const V3 cb = curl_from_pairs(qb<1>(e1), qb<1>(e2), qb<2>(e1), qb<2>(e2), qb<3>(e1),
                                 qb<3>(e2), i2d);
-  const double bb = dot3(B, B);
-  const double ibm = rsqrt_nr(bb);
-  const double Bmag = bb * ibm;
-  const V3 ubc{B.x * ibm, B.y * ibm, B.z * ibm};
-  double o0, o1, o2, o3;
-  if (eom == EOM_NT) {
-    const double gamma = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
-    const double gm = gamma * mass;
-    const double coef = (gm * (v * v) + (Y3 * Y3) / gm) / (2.0 * charge * (Bmag * Bmag));
-    const V3 c = cross3(ubc, gB);
-    const double pg = Y3 / gm;
-    o0 = coef * c.x + pg * ubc.x;
-    o1 = coef * c.y + pg * ubc.y;
-    o2 = coef * c.z + pg * ubc.z;
-    o3 = -mu * dot3(ubc, gB) / gamma;
-  } else {
-    const double iq = rcp_nr(charge);
-    const V3 Bs{B.x + Y3 * cb.x * iq, B.y + Y3 * cb.y * iq, B.z + Y3 * cb.z * iq};
-    const double Bspar = dot3(Bs, ubc);
-    const double iBs = rcp_nr(Bspar);
-    if (eom == EOM_BC) {
-      const double gamma = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
-      const double a = Y3 / (gamma * mass);
-      const double bcoef = mu / (charge * gamma);
-      const V3 c = cross3(ubc, gB);
-      o0 = (a * Bs.x + bcoef * c.x) * iBs;
-      o1 = (a * Bs.y + bcoef * c.y) * iBs;
-      o2 = (a * Bs.z + bcoef * c.z) * iBs;
-      o3 = -mu * dot3(Bs, gB) / (gamma * Bspar);
-    } else {
-      const double imc = rcp_nr(mass * C_LIGHT);
-      const double pr = Y3 * imc;
-      const double gamma =
-          __builtin_sqrt(1.0 + 2.0 * mu * Bmag * (imc * (1.0 / C_LIGHT)) + pr * pr);
-      const double igam = rcp_nr(gamma);
-      const double mg = mu * igam;
-      V3 Es;
-      if constexpr (has_E<M>()) {
-        const V3 E = fieldE<M>(F);
-        Es = V3{E.x - mg * gB.x * iq, E.y - mg * gB.y * iq, E.z - mg * gB.z * iq};
-      } else {
-        Es = V3{-mg * gB.x * iq, -mg * gB.y * iq, -mg * gB.z * iq};
-      }
-      if (!F.is_static) {
-        // lane 0's second point was b(t - tstep); b(t + tstep) is a third evaluation
-        const V3 Bc = fieldB<M>(F, t + F.tstep, Y0, Y1, Y2);
-        const double ic = rsqrt_nr(dot3(Bc, Bc));
-        const double s = 0.5 / F.tstep;
-        const double l0 = (Bc.x * ic - ub_.x) * s, l1 = (Bc.y * ic - ub_.y) * s,
-                     l2 = (Bc.z * ic - ub_.z) * s;
-        const V3 db{qb<0>(l0), qb<0>(l1), qb<0>(l2)};
-        Es.x -= Y3 * db.x * iq;
-        Es.y -= Y3 * db.y * iq;
-        Es.z -= Y3 * db.z * iq;
-      }
-      const double a = pr * C_LIGHT * igam;
-      const V3 c = cross3(Es, ubc);
-      o0 = (a * Bs.x + c.x) * iBs;
-      o1 = (a * Bs.y + c.y) * iBs;
-      o2 = (a * Bs.z + c.z) * iBs;
-      o3 = charge * dot3(Es, Bs) * iBs;
-    }
+  V3 db{0.0, 0.0, 0.0};
+  if (eom == EOM_TCB && !F.is_static) {
+    // lane 0's second point was B(t - tstep); B(t + tstep) is a third evaluation
+    const V3 Bc = fieldB<M>(F, t + F.tstep, Y0, Y1, Y2);
+    const double ic = rsqrt_nr(dot3(Bc, Bc));
+    const double s = 0.5 / F.tstep;
+    const double l0 = (Bc.x * ic - ub_.x) * s, l1 = (Bc.y * ic - ub_.y) * s,
+                 l2 = (Bc.z * ic - ub_.z) * s;
+    db = V3{qb<0>(l0), qb<0>(l1), qb<0>(l2)};
   }
-  if (eq) {
-    o2 = 0.0;
-    o3 = 0.0;
-  }
-  return sel4(q, o0, o1, o2, o3);
+  const double Y[4] = {Y0, Y1, Y2, Y3};
+  double o[4];
+  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, eq, Y, B, gB,
+             cb, db, o);
+  return sel4(q, o[0], o[1], o[2], o[3]);
 }
 
 // resident waves per SIMD the register allocation targets

@@ -17,6 +17,26 @@ namespace rapt {
 
 constexpr double UROUND = 2.3e-16;
 
+// Magnitude that scales component i in the error norms.  G = false: |a_i|,
+// scipy's componentwise sc_i = atol + rtol max(|y_i|, |y1_i|).  G = true
+// (FP32-field mode): the largest |a_j| of the component's 3-vector (x, y, z |
+// momentum | ppar), so a component passing through zero, like z at the
+// equator, keeps its vector's scale: with float forces the rounding noise of
+// a step is ~1e-7 of |v| h, which a componentwise scale rtol |z| ~ rtol |v| h
+// near z = 0 can never accept.
+template <int N, bool G>
+__device__ __forceinline__ double err_mag(const double* a, int i) {
+  if constexpr (!G) {
+    return __builtin_fabs(a[i]);
+  } else {
+    const int g0 = (i / 3) * 3;
+    double m = __builtin_fabs(a[g0]);
+    if (g0 + 1 < N) m = fmax(m, __builtin_fabs(a[g0 + 1]));
+    if (g0 + 2 < N) m = fmax(m, __builtin_fabs(a[g0 + 2]));
+    return m;
+  }
+}
+
 // ---- DOPRI5 -------------------------------------------------------------------
 namespace dp5 {
 constexpr double c2 = 0.2, c3 = 0.3, c4 = 0.8, c5 = 8.0 / 9.0;
@@ -34,7 +54,7 @@ constexpr double e1 = 71.0 / 57600.0, e3 = -71.0 / 16695.0, e4 = 71.0 / 1920.0,
 
 // One DOPRI5 step attempt from (x, y) with k1 = f(x, y).  Writes the 5th-order
 // solution to y1 and f(x+h, y1) to k7; returns the scaled error norm.
-template <int N, class RHS>
+template <int N, bool G = false, class RHS>
 __device__ __forceinline__ double dopri5_attempt(RHS& f, double x, double h, const double* y,
                                                  const double* k1, double* y1, double* k7,
                                                  double atol, double rtol) {
@@ -100,7 +120,7 @@ __device__ __forceinline__ double dopri5_attempt(RHS& f, double x, double h, con
   for (int i = 0; i < N; ++i) {
     const double e = (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] +
                       e7 * k7[i]) * h;
-    const double sk = atol + rtol * fmax(__builtin_fabs(y[i]), __builtin_fabs(y1[i]));
+    const double sk = atol + rtol * fmax(err_mag<N, G>(y, i), err_mag<N, G>(y1, i));
     const double q = e / sk;
     err = __builtin_fma(q, q, err);
   }
@@ -187,7 +207,7 @@ constexpr double er1 = 0.1312004499419488073250102996e-01,
 // One DOP853 step attempt (11 stage evaluations).  Writes the 8th-order
 // solution to y1; returns the combined 5th/3rd-order error norm.  The caller
 // evaluates f(x+h, y1) after acceptance (the 12th evaluation).
-template <int N, class RHS>
+template <int N, bool G = false, class RHS>
 __device__ __forceinline__ double dop853_attempt(RHS& f, double x, double h, const double* y,
                                                  const double* k1, double* y1, double atol,
                                                  double rtol) {
@@ -244,7 +264,7 @@ __device__ __forceinline__ double dop853_attempt(RHS& f, double x, double h, con
     const double bs = b1 * k1[i] + b6 * k6[i] + b7 * k7[i] + b8 * k8[i] + b9 * k9[i] +
                       b10 * k10[i] + b11 * k2[i] + b12 * k3[i];
     y1[i] = y[i] + h * bs;
-    const double sk = atol + rtol * fmax(__builtin_fabs(y[i]), __builtin_fabs(y1[i]));
+    const double sk = atol + rtol * fmax(err_mag<N, G>(y, i), err_mag<N, G>(y1, i));
     const double isk = 1.0 / sk;
     const double e3 = (bs - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i]) * isk;
     err2 += e3 * e3;
@@ -258,14 +278,14 @@ __device__ __forceinline__ double dop853_attempt(RHS& f, double x, double h, con
 }
 
 // ---- HINIT (initial step guess) ----------------------------------------------
-template <int N, class RHS>
+template <int N, bool G = false, class RHS>
 __device__ __forceinline__ double hinit(RHS& f, double x, const double* y, const double* f0,
                                         int iord, double hmax, double atol, double rtol) {
   double dnf = 0.0, dny = 0.0;
   double sk[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    sk[i] = atol + rtol * __builtin_fabs(y[i]);
+    sk[i] = atol + rtol * err_mag<N, G>(y, i);
     const double a = f0[i] / sk[i], b = y[i] / sk[i];
     dnf = __builtin_fma(a, a, dnf);
     dny = __builtin_fma(b, b, dny);

@@ -336,6 +336,66 @@ def test_work_order_does_not_change_results(lpp):
         np.testing.assert_array_equal(x, y)
 
 
+def test_fp32_field_particle_tracks_fp64():
+    """FP32-field Particle mode (field and force in float, state and error
+    control in double, I-controller) against FP64 with the same controller
+    (beta = 0) over 1 s of 1 MeV protons in EarthDipole: positions within
+    1e-5 relative and |p| drift below 1e-5.  Float rounding of the force
+    (~6e-8 per evaluation) accumulates over the run."""
+    wl = W.dipole_protons_1mev(n=64)
+    m, q = K.m_pr, K.e
+    g = 1 / np.sqrt(1 - np.sum(wl["vel"] ** 2, axis=1) / K.c ** 2)
+    last = np.zeros((64, 7))
+    last[:, 1:4] = wl["pos"]
+    last[:, 4:7] = (m * g)[:, None] * wl["vel"]
+    a = batch.advance_particles(F.EarthDipole(), last, m, q, 1.0, params=P(), beta=0.0)
+    b = batch.advance_particles(F.EarthDipole(), last, m, q, 1.0, params=P(),
+                                precision="fp32_field")
+    assert np.all(b.status == N.ST_DONE), np.unique(b.status, return_counts=True)
+    worst_pos = worst_e = 0.0
+    for i, (x, y) in enumerate(zip(a.rows, b.rows)):
+        assert x.shape == y.shape
+        worst_pos = max(worst_pos, pos_err(y, x))
+        p0 = np.linalg.norm(last[i, 4:7])
+        worst_e = max(worst_e, np.max(np.abs(np.linalg.norm(y[:, 4:7], axis=1) - p0)) / p0)
+    print("fp32 particle: pos %.3g, |p| drift %.3g" % (worst_pos, worst_e))
+    assert worst_pos < 1e-5
+    assert worst_e < 1e-5
+
+
+def test_fp32_field_gc_tracks_fp64():
+    """FP32-field guiding centres (field and finite-difference drift terms in
+    float with step |r|/1024, algebra and state in double, I-controller)
+    against FP64 with the same controller over 1 s of DoubleDipole electrons:
+    positions within 1e-4 relative."""
+    f = F.DoubleDipole()
+    n = 64
+    last, mu, v = _gc_batch(f, n)
+    prm = P(solvertolerances=(1e-8, 1e-8))
+    a = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, beta=-1.0)
+    b = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm,
+                          precision="fp32_field")
+    assert np.all(b.status == N.ST_DONE), np.unique(b.status, return_counts=True)
+    worst = max(pos_err(y, x) for x, y in zip(a.rows, b.rows))
+    print("fp32 gc: pos %.3g" % worst)
+    assert worst < 1e-4
+
+
+def test_precision_rejects_unknown_value():
+    d = N.make_desc(N.MODE_GC, P(), 1.0)
+    d.precision = 7
+    ctx = N.context()
+    ctx.set_field(F.DoubleDipole().descriptor())
+    b = N.Batch(ctx, 1)
+    try:
+        b.upload(np.zeros((N.NSLOT, 1)), np.zeros(1, np.int32))
+        b.set_rows(5, np.array([4], np.int64))
+        with pytest.raises(RuntimeError, match="precision"):
+            b.run(d)
+    finally:
+        b.close()
+
+
 def test_work_refill_is_exact():
     """Fewer resident lanes than particles (work refill) gives the same rows."""
     wl = W.dipole_protons_1mev(n=200)
