@@ -157,6 +157,62 @@ int rapt_field_eval(rapt_ctx *ctx, int what, int64_t n, const double *tpos, doub
 int rapt_halfbounce(rapt_ctx *ctx, int64_t n, const double *tpos, const double *Bm,
                     double fieldlineresolution, double *sb, int32_t *status);
 
+/* Surrogate labels, one bounce-averaged drift evaluation per query: replaces
+ * BounceCenter(pos, v, 0, pa, mass, charge, field) followed by the drift of
+ * its deriv (rapt/BounceCenter.py:74-115, 235-244), flutils.bounceperiod
+ * (rapt/flutils.py:232-252) and flutils.gradI / eye (rapt/flutils.py:65-229).
+ * query[3i..3i+2] = (L [Re], equatorial pitch angle [deg], log10 K [eV]), the
+ * particle starting on the x axis at L Re in the equatorial plane;
+ * out[3i..3i+2] = (bounce period [s], bounce-averaged drift angular velocity
+ * about z [rad/s], +b mirror-point latitude [deg]).  Re and eV are the
+ * caller's constants (rapt/__init__.py:4-10).  Static dipole-type fields
+ * (models 1, 2, 5, 8).  status[i] as rapt_halfbounce. */
+int rapt_bounce_labels(rapt_ctx *ctx, int64_t n, const double *query, double mass,
+                       double charge, double Re, double eV, double eyegradientstep,
+                       double fieldlineresolution, double *out, int32_t *status);
+
+/* ---- bounce-drift surrogate (no counterpart in the reference) ------------------
+ * An ensemble of MLPs 3 -> 256 -> 256 -> 256 -> 3 (SiLU; BF16 weights, FP32
+ * accumulation; hidden 256 x 256 layers on MFMA) that maps a query (L [Re],
+ * equatorial pitch angle [deg], log10 K [eV]) to (bounce period, |bounce-
+ * averaged drift angular velocity|, mirror latitude) of rapt_bounce_labels.
+ * Outputs are learned as normalised log-residuals against dipole closed
+ * forms (Walt's bounce and drift periods, rapt/utils.py:435-513, and the
+ * exact dipole mirror latitude).  Parameters per member: RAPT_MLP_NPARAM
+ * floats, W1[256][3], b1[256], W2[256][256] (out, in), b2, W3, b3, W4[3][256],
+ * b4[3]. */
+#define RAPT_MLP_NPARAM 133379
+typedef struct rapt_mlp rapt_mlp;
+
+/* members (1..4) ensemble members from theta0 [members][RAPT_MLP_NPARAM]; mass [kg]
+ * and eV [J] set the species and units of the dipole baselines. */
+int rapt_mlp_create(rapt_ctx *ctx, int32_t members, const float *theta0, double mass,
+                    double eV, rapt_mlp **out);
+int rapt_mlp_destroy(rapt_mlp *m);
+/* output normalisation: shift[3] then scale[3] (scale > 0) of the residuals */
+int rapt_mlp_set_norm(rapt_mlp *m, const float *shift_scale);
+/* features X [n][3] and targets T [n][3] from queries [n][3] and traced labels
+ * [n][3] (rapt_bounce_labels); normalised != 0 applies the shift/scale. */
+int rapt_mlp_targets(rapt_mlp *m, int64_t n, const float *query, const double *labels,
+                     int normalised, float *X, float *T);
+/* steps Adam steps (all members together) on mean squared error, batch rows
+ * per step taken from order[steps * batch] (indices into X/T); learning rate
+ * cosine-decayed from lr0 to lr1 over the call (lr0 = 0: gradients only).
+ * loss_out [members]: last step's loss; grad_out [members][NPARAM]: last
+ * step's gradients (either may be NULL).  batch: a multiple of 64. */
+int rapt_mlp_train(rapt_mlp *m, int64_t n, const float *X, const float *T, int32_t batch,
+                   int32_t steps, const int64_t *order, float lr0, float lr1,
+                   double *loss_out, float *grad_out);
+int rapt_mlp_get_params(rapt_mlp *m, float *theta);
+/* training-path forward of features X [n][3]: Y [members][n][3] normalised */
+int rapt_mlp_forward(rapt_mlp *m, int64_t n, const float *X, float *Y);
+/* fused ensemble inference: out [n][3] physical values (ensemble mean of the
+ * normalised outputs), flag[i] = 1 where a member deviates from the
+ * ensemble mean by more than thresh (relative) on any output (the caller traces those: rapt_bounce_labels);
+ * raw [members][n][3] optional; kernel_ms: inference kernel time. */
+int rapt_mlp_infer(rapt_mlp *m, int64_t n, const float *query, float thresh, float *out,
+                   int32_t *flag, float *raw, double *kernel_ms);
+
 /* ---- device-resident batch (replaces one scipy ode loop per particle) -- */
 int rapt_batch_create(rapt_ctx *ctx, int64_t n, rapt_batch **out);
 int rapt_batch_destroy(rapt_batch *b);

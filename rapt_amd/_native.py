@@ -85,6 +85,18 @@ def lib():
     L.rapt_field_set.argtypes = [vp, C.POINTER(FieldDesc)]
     L.rapt_field_eval.argtypes = [vp, C.c_int, C.c_int64, dp, dp]
     L.rapt_halfbounce.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
+    L.rapt_bounce_labels.argtypes = [vp, C.c_int64, dp, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, dp, i32p]
+    fp = C.POINTER(C.c_float)
+    L.rapt_mlp_create.argtypes = [vp, C.c_int32, fp, C.c_double, C.c_double, C.POINTER(vp)]
+    L.rapt_mlp_destroy.argtypes = [vp]
+    L.rapt_mlp_set_norm.argtypes = [vp, fp]
+    L.rapt_mlp_targets.argtypes = [vp, C.c_int64, fp, dp, C.c_int, fp, fp]
+    L.rapt_mlp_train.argtypes = [vp, C.c_int64, fp, fp, C.c_int32, C.c_int32, i64p,
+                                 C.c_float, C.c_float, dp, fp]
+    L.rapt_mlp_get_params.argtypes = [vp, fp]
+    L.rapt_mlp_forward.argtypes = [vp, C.c_int64, fp, fp]
+    L.rapt_mlp_infer.argtypes = [vp, C.c_int64, fp, C.c_float, fp, i32p, fp, dp]
     L.rapt_batch_create.argtypes = [vp, C.c_int64, C.POINTER(vp)]
     L.rapt_batch_destroy.argtypes = [vp]
     L.rapt_batch_upload.argtypes = [vp, dp, i32p]
@@ -95,7 +107,9 @@ def lib():
     L.rapt_trace.argtypes = [vp, C.POINTER(TraceDesc), C.c_int64, dp, i32p, i64p, dp, i64p,
                              i32p, C.POINTER(Counters)]
     for nm in ("rapt_ctx_create", "rapt_ctx_destroy", "rapt_field_set", "rapt_field_eval",
-               "rapt_halfbounce",
+               "rapt_halfbounce", "rapt_bounce_labels", "rapt_mlp_create", "rapt_mlp_destroy",
+               "rapt_mlp_set_norm", "rapt_mlp_targets", "rapt_mlp_train", "rapt_mlp_get_params",
+               "rapt_mlp_forward", "rapt_mlp_infer",
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
                "rapt_batch_reset"):
@@ -172,6 +186,22 @@ class Context:
                                          float(fieldlineresolution), _dp(sb), _ip32(st)),
                    "rapt_halfbounce")
         return sb, st
+
+
+    def bounce_labels(self, desc, query, mass, charge, eyegradientstep, fieldlineresolution):
+        """(T_b, omega_d, mirror latitude) per (L, alpha_eq [deg], log10 K [eV]) query;
+        returns (out (n, 3), status)."""
+        from .constants import Re, e
+        self.set_field(desc)
+        query = np.ascontiguousarray(np.atleast_2d(query), dtype=np.float64)
+        n = query.shape[0]
+        out = np.zeros((n, 3))
+        st = np.zeros(n, np.int32)
+        self.check(lib().rapt_bounce_labels(self.h, n, _dp(query), float(mass), float(charge),
+                                            float(Re), float(e), float(eyegradientstep),
+                                            float(fieldlineresolution), _dp(out), _ip32(st)),
+                   "rapt_bounce_labels")
+        return out, st
 
 
 def context(device=0):

@@ -56,6 +56,8 @@ struct BounceArgs {
   double* Bv;    // [2*cap][lanes] |B|
   double* Cc;    // [2*cap][lanes] spline coefficients
   double* Wk;    // [2*cap][lanes] elimination workspace
+  double* Zs;    // [2*cap][lanes] z of the trace points (labels only, else nullptr)
+  double* Rs;    // [2*cap][lanes] cylindrical radius of the trace points (labels only)
   double* out;
   int32_t* status;
 };
@@ -93,6 +95,10 @@ __device__ int line_chunk(const BounceArgs& A, int64_t lane, double t, double si
       const int64_t k = base + dir * m;
       A.S[k * A.lanes + lane] = Y[0];
       A.Bv[k * A.lanes + lane] = magB<M>(A.F, t, Y[1], Y[2], Y[3]);
+      if (A.Zs) {
+        A.Zs[k * A.lanes + lane] = Y[3];
+        A.Rs[k * A.lanes + lane] = __builtin_sqrt(Y[1] * Y[1] + Y[2] * Y[2]);
+      }
     }
     h = h * fmin(fmax(0.84 * pow(tol / r, 0.25), 0.1), 4.0);
     if (h > ds) {
@@ -197,115 +203,251 @@ __device__ bool brent(Fn& f, double a, double b, double& root) {
   return true;
 }
 
+// Integrals over the field line through (x0, y0, z0) between the mirror
+// points of B = Bm: sb = int ds / sqrt(1 - B/Bm) (flutils.halfbouncepath,
+// rapt/flutils.py:254-316), eye = int sqrt(1 - B/Bm) ds (flutils.eye,
+// rapt/flutils.py:65-151; returns 0 where the reference does: Bm below the
+// minimum of B on the line, or an equatorial particle to 1e-12), and lat =
+// latitude (rad) of the mirror point along +b (when A.Zs is set).  The eye
+// integral uses the same quadratic spline and Gauss-Legendre quadrature as
+// sb for every pitch angle; the reference switches to Simpson's rule on the
+// trace points below 70 degrees, a rule whose discretisation error depends on
+// where the trace points fall and so adds jitter to finite differences of I.
+template <int M>
+__device__ int32_t line_integrals(const BounceArgs& A, int64_t lane, double t, double x0,
+                                  double y0, double z0, double Bm, double* sb_out,
+                                  double* eye_out, double* lat_out) {
+  const FieldC& F = A.F;
+  const int32_t base = A.cap;  // array index of the start point
+  const double ds = 1.0 / curvature<M>(F, t, x0, y0, z0) / A.res;
+  A.S[(int64_t)base * A.lanes + lane] = 0.0;
+  A.Bv[(int64_t)base * A.lanes + lane] = magB<M>(F, t, x0, y0, z0);
+  if (A.Zs) {
+    A.Zs[(int64_t)base * A.lanes + lane] = z0;
+    A.Rs[(int64_t)base * A.lanes + lane] = __builtin_sqrt(x0 * x0 + y0 * y0);
+  }
+  int32_t st = BST_OK;
+  int32_t nf = 0, nb = 0;
+  for (int side = 0; side < 2 && st == BST_OK; ++side) {
+    const double sign = side == 0 ? 1.0 : -1.0;
+    const double tol = side == 0 ? 1e-3 : 1e-4;
+    const int32_t dir = side == 0 ? 1 : -1;
+    int32_t* np = side == 0 ? &nf : &nb;
+    double Y[4] = {0.0, x0, y0, z0};
+    for (;;) {
+      const int rc = line_chunk<M>(A, lane, t, sign, tol, ds, Y, np, base, dir);
+      if (rc < 0) { st = BST_OVERFLOW; break; }
+      if (rc == 0) break;
+      if (magB<M>(F, t, Y[1], Y[2], Y[3]) > Bm) break;
+    }
+  }
+  double sb = __builtin_nan(""), eye = __builtin_nan(""), lat = __builtin_nan("");
+  if (st == BST_OK) {
+    // curve = array indices [base-nb, base+nf]; keep B <= Bm plus one each side
+    const int32_t lo = base - nb, hi = base + nf;
+    int32_t first = -1, lastin = -1;
+    double bmin = A.Bv[(int64_t)lo * A.lanes + lane];
+    for (int32_t k = lo; k <= hi; ++k) {
+      const double bk = A.Bv[(int64_t)k * A.lanes + lane];
+      bmin = fmin(bmin, bk);
+      if (bk <= Bm) {
+        if (first < 0) first = k;
+        lastin = k;
+      }
+    }
+    int32_t i1, i2;
+    const int32_t ncur = hi - lo + 1;
+    if (first < 0) {
+      i1 = lo + (ncur - 3) / 2;
+      i2 = lo + (ncur + 1) / 2;
+    } else {
+      i1 = first - 1 < lo ? lo : first - 1;
+      i2 = lastin + 1 > hi ? hi : lastin + 1;
+    }
+    const int32_t nt = i2 - i1 + 1;
+    auto X = [&](int32_t j) { return A.S[(int64_t)(i1 + j) * A.lanes + lane]; };
+    auto Yv = [&](int32_t j) { return A.Bv[(int64_t)(i1 + j) * A.lanes + lane]; };
+    const bool eye_zero = bmin > Bm || __builtin_fabs(bmin - Bm) / Bm < 1e-12;
+    if (nt == 3) {
+      // equatorial: B(s) ~ B(s0) + B''/2 (s - s0)^2 from the Lagrange parabola
+      const double s12 = X(0) - X(1), s23 = X(1) - X(2), s13 = X(0) - X(2);
+      const double B2s = 2 * (Yv(0) * s23 - Yv(1) * s13 + Yv(2) * s12) / (s12 * s13 * s23);
+      sb = 3.141592653589793 * __builtin_sqrt(2 * Bm / B2s);
+      const double a2 = fmax(2.0 * (Bm - Yv(1)) / B2s, 0.0);
+      eye = eye_zero ? 0.0 : 0.5 * 3.141592653589793 * a2 * __builtin_sqrt(B2s / (2.0 * Bm));
+      if (A.Zs) {
+        const double r1 = A.Rs[(int64_t)(i1 + 1) * A.lanes + lane];
+        lat = atan(__builtin_sqrt(a2) / r1);
+      }
+    } else if (nt < 3) {
+      st = BST_NOBRACKET;
+    } else {
+      Spline sp{A.S, A.Cc, A.lanes, lane, i1, nt};
+      // tridiagonal collocation system (row j: N_{j-1}, N_j, N_{j+1} at x_j)
+      double cprev = 0.0, dprev = Yv(0);
+      A.Wk[(int64_t)i1 * A.lanes + lane] = 0.0;
+      A.Cc[(int64_t)i1 * A.lanes + lane] = dprev;
+      for (int32_t j = 1; j < nt; ++j) {
+        double sub, dia, sup;
+        if (j == nt - 1) {
+          sub = 0.0; dia = 1.0; sup = 0.0;
+        } else {
+          double b[3];
+          sp.basis(j + 1, X(j), b);
+          sub = b[0]; dia = b[1]; sup = b[2];
+        }
+        const double m = dia - sub * cprev;
+        cprev = sup / m;
+        dprev = (Yv(j) - sub * dprev) / m;
+        A.Wk[(int64_t)(i1 + j) * A.lanes + lane] = cprev;
+        A.Cc[(int64_t)(i1 + j) * A.lanes + lane] = dprev;
+      }
+      for (int32_t j = nt - 2; j >= 0; --j) {
+        const int64_t kj = (int64_t)(i1 + j) * A.lanes + lane;
+        const int64_t kn = (int64_t)(i1 + j + 1) * A.lanes + lane;
+        A.Cc[kj] = A.Cc[kj] - A.Wk[kj] * A.Cc[kn];
+      }
+      auto g = [&](double s) { return sp.eval(s) - Bm; };
+      double sm1, sm2;
+      if (!brent(g, X(0), X(1), sm1) || !brent(g, X(nt - 2), X(nt - 1), sm2)) {
+        st = BST_ROOT;
+      } else {
+        // s = mid - half cos(theta) removes the end-point singularities
+        const double mid = 0.5 * (sm1 + sm2), half = 0.5 * (sm2 - sm1);
+        const int P = 32;
+        const double pw = 3.141592653589793 / P;
+        double acc = 0.0, acc_i = 0.0;
+        for (int p = 0; p < P; ++p) {
+          const double c0 = (p + 0.5) * pw;
+          for (int q = 0; q < 16; ++q) {
+            const double xg = (q < 8) ? -GL16_X[7 - q] : GL16_X[q - 8];
+            const double wg = (q < 8) ? GL16_W[7 - q] : GL16_W[q - 8];
+            const double th = c0 + 0.5 * pw * xg;
+            const double s = mid - half * cos(th);
+            const double val = 1.0 - sp.eval(s) / Bm;
+            const double sn = sin(th);
+            acc += wg * sn / __builtin_sqrt(val);
+            if (eye_out) acc_i += wg * sn * __builtin_sqrt(fmax(val, 0.0));
+          }
+        }
+        sb = acc * 0.5 * pw * half;
+        eye = eye_zero ? 0.0 : acc_i * 0.5 * pw * half;
+        if (!__builtin_isfinite(sb)) st = BST_NAN;
+        if (A.Zs) {
+          // quadratic Lagrange interpolation of (z, rho) at the +b mirror point
+          double zz = 0.0, rr = 0.0;
+          for (int a = 0; a < 3; ++a) {
+            const int32_t ja = nt - 3 + a;
+            double w = 1.0;
+            for (int b2 = 0; b2 < 3; ++b2)
+              if (b2 != a) w *= (sm2 - X(nt - 3 + b2)) / (X(ja) - X(nt - 3 + b2));
+            zz += w * A.Zs[(int64_t)(i1 + ja) * A.lanes + lane];
+            rr += w * A.Rs[(int64_t)(i1 + ja) * A.lanes + lane];
+          }
+          lat = atan2(zz, rr);
+        }
+      }
+    }
+  }
+  if (sb_out) *sb_out = sb;
+  if (eye_out) *eye_out = eye;
+  if (lat_out) *lat_out = lat;
+  return st;
+}
+
 template <int M>
 __global__ __launch_bounds__(256) void k_halfbounce(BounceArgs A) {
   const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= A.lanes) return;
-  const FieldC& F = A.F;
-  const int32_t base = A.cap;  // array index of the start point
   for (int64_t i = lane; i < A.n; i += A.lanes) {
-    const double t = A.tpos[4 * i];
-    const double x0 = A.tpos[4 * i + 1], y0 = A.tpos[4 * i + 2], z0 = A.tpos[4 * i + 3];
-    const double Bm = A.Bm[i];
-    const double ds = 1.0 / curvature<M>(F, t, x0, y0, z0) / A.res;
-    A.S[(int64_t)base * A.lanes + lane] = 0.0;
-    A.Bv[(int64_t)base * A.lanes + lane] = magB<M>(F, t, x0, y0, z0);
-    int32_t st = BST_OK;
-    int32_t nf = 0, nb = 0;
-    for (int side = 0; side < 2 && st == BST_OK; ++side) {
-      const double sign = side == 0 ? 1.0 : -1.0;
-      const double tol = side == 0 ? 1e-3 : 1e-4;
-      const int32_t dir = side == 0 ? 1 : -1;
-      int32_t* np = side == 0 ? &nf : &nb;
-      double Y[4] = {0.0, x0, y0, z0};
-      for (;;) {
-        const int rc = line_chunk<M>(A, lane, t, sign, tol, ds, Y, np, base, dir);
-        if (rc < 0) { st = BST_OVERFLOW; break; }
-        if (rc == 0) break;
-        if (magB<M>(F, t, Y[1], Y[2], Y[3]) > Bm) break;
-      }
-    }
-    double sb = __builtin_nan("");
-    if (st == BST_OK) {
-      // curve = array indices [base-nb, base+nf]; keep B <= Bm plus one each side
-      const int32_t lo = base - nb, hi = base + nf;
-      int32_t first = -1, lastin = -1;
-      for (int32_t k = lo; k <= hi; ++k) {
-        if (A.Bv[(int64_t)k * A.lanes + lane] <= Bm) {
-          if (first < 0) first = k;
-          lastin = k;
-        }
-      }
-      int32_t i1, i2;
-      const int32_t ncur = hi - lo + 1;
-      if (first < 0) {
-        i1 = lo + (ncur - 3) / 2;
-        i2 = lo + (ncur + 1) / 2;
-      } else {
-        i1 = first - 1 < lo ? lo : first - 1;
-        i2 = lastin + 1 > hi ? hi : lastin + 1;
-      }
-      const int32_t nt = i2 - i1 + 1;
-      auto X = [&](int32_t j) { return A.S[(int64_t)(i1 + j) * A.lanes + lane]; };
-      auto Yv = [&](int32_t j) { return A.Bv[(int64_t)(i1 + j) * A.lanes + lane]; };
-      if (nt == 3) {
-        const double s12 = X(0) - X(1), s23 = X(1) - X(2), s13 = X(0) - X(2);
-        const double B2s = 2 * (Yv(0) * s23 - Yv(1) * s13 + Yv(2) * s12) / (s12 * s13 * s23);
-        sb = 3.141592653589793 * __builtin_sqrt(2 * Bm / B2s);
-      } else if (nt < 3) {
-        st = BST_NOBRACKET;
-      } else {
-        Spline sp{A.S, A.Cc, A.lanes, lane, i1, nt};
-        // tridiagonal collocation system (row j: N_{j-1}, N_j, N_{j+1} at x_j)
-        double cprev = 0.0, dprev = Yv(0);
-        A.Wk[(int64_t)i1 * A.lanes + lane] = 0.0;
-        A.Cc[(int64_t)i1 * A.lanes + lane] = dprev;
-        for (int32_t j = 1; j < nt; ++j) {
-          double sub, dia, sup;
-          if (j == nt - 1) {
-            sub = 0.0; dia = 1.0; sup = 0.0;
-          } else {
-            double b[3];
-            sp.basis(j + 1, X(j), b);
-            sub = b[0]; dia = b[1]; sup = b[2];
-          }
-          const double m = dia - sub * cprev;
-          cprev = sup / m;
-          dprev = (Yv(j) - sub * dprev) / m;
-          A.Wk[(int64_t)(i1 + j) * A.lanes + lane] = cprev;
-          A.Cc[(int64_t)(i1 + j) * A.lanes + lane] = dprev;
-        }
-        for (int32_t j = nt - 2; j >= 0; --j) {
-          const int64_t kj = (int64_t)(i1 + j) * A.lanes + lane;
-          const int64_t kn = (int64_t)(i1 + j + 1) * A.lanes + lane;
-          A.Cc[kj] = A.Cc[kj] - A.Wk[kj] * A.Cc[kn];
-        }
-        auto g = [&](double s) { return sp.eval(s) - Bm; };
-        double sm1, sm2;
-        if (!brent(g, X(0), X(1), sm1) || !brent(g, X(nt - 2), X(nt - 1), sm2)) {
-          st = BST_ROOT;
-        } else {
-          const double mid = 0.5 * (sm1 + sm2), half = 0.5 * (sm2 - sm1);
-          const int P = 32;
-          const double pw = 3.141592653589793 / P;
-          double acc = 0.0;
-          for (int p = 0; p < P; ++p) {
-            const double c0 = (p + 0.5) * pw;
-            for (int q = 0; q < 16; ++q) {
-              const double xg = (q < 8) ? -GL16_X[7 - q] : GL16_X[q - 8];
-              const double wg = (q < 8) ? GL16_W[7 - q] : GL16_W[q - 8];
-              const double th = c0 + 0.5 * pw * xg;
-              const double s = mid - half * cos(th);
-              const double val = 1.0 - sp.eval(s) / Bm;
-              acc += wg * sin(th) / __builtin_sqrt(val);
-            }
-          }
-          sb = acc * 0.5 * pw * half;
-          if (!__builtin_isfinite(sb)) st = BST_NAN;
-        }
-      }
-    }
+    double sb;
+    const int32_t st = line_integrals<M>(A, lane, A.tpos[4 * i], A.tpos[4 * i + 1],
+                                         A.tpos[4 * i + 2], A.tpos[4 * i + 3], A.Bm[i], &sb,
+                                         nullptr, nullptr);
     A.out[i] = sb;
+    A.status[i] = st;
+  }
+}
+
+// Surrogate labels from one bounce-averaged drift evaluation per query
+// (BounceCenter.advance's deriv, rapt/BounceCenter.py:235-244, at t = 0):
+// query (L, equatorial pitch angle alpha [deg], log10 K [eV]) starts at the
+// equatorial point (L Re, 0, 0); v from K (utils.speedfromKE), Bm =
+// B_eq / sin^2 alpha, T_b = 2 S_b / v (flutils.bounceperiod), grad I by
+// flutils.gradI (rapt/flutils.py:153-229: central differences of eye across
+// the field, step eyegradientstep, one-sided where an I is 0), v_d =
+// gamma m v^2 / (q S_b B^2) grad I x B, omega_d = v_d . phi_hat / rho.
+// out[3 i + 0..2] = (T_b [s], omega_d [rad/s], mirror latitude [deg]).
+struct LabelArgs {
+  double mass, charge, dstep;
+  double Re, eV;  // Earth radius [m] and joules per eV, as the caller's constants
+};
+
+template <int M>
+__global__ __launch_bounds__(256) void k_bounce_labels(BounceArgs A, LabelArgs L) {
+  const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= A.lanes) return;
+  const FieldC& F = A.F;
+  const double Re = L.Re;
+  const double mc2 = L.mass * C_LIGHT * C_LIGHT;
+  for (int64_t i = lane; i < A.n; i += A.lanes) {
+    const double Lsh = A.tpos[3 * i], alpha = A.tpos[3 * i + 1] * (3.141592653589793 / 180.0);
+    const double K = pow(10.0, A.tpos[3 * i + 2]) * L.eV;
+    // speedfromKE (rapt/utils.py:218-249): non-relativistic below K/mc2 = 1e-6
+    double v;
+    if (K / mc2 < 1e-6) {
+      v = __builtin_sqrt(2.0 * K / L.mass);
+    } else {
+      const double g = 1.0 + K / mc2;
+      v = C_LIGHT * __builtin_sqrt(1.0 - 1.0 / (g * g));
+    }
+    const double gamma = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
+    const double x0 = Lsh * Re, y0 = 0.0, z0 = 0.0, t = 0.0;
+    const V3 B = fieldB<M>(F, t, x0, y0, z0);
+    const double bsq = dot3(B, B);
+    const double sa = sin(alpha);
+    const double Bm = __builtin_sqrt(bsq) / (sa * sa);
+    double sb, lat, ic;
+    int32_t st = line_integrals<M>(A, lane, t, x0, y0, z0, Bm, &sb, &ic, &lat);
+    // gradI directions (rapt/flutils.py:196-207)
+    const double mx = fmax(__builtin_fabs(x0), __builtin_fabs(y0));
+    V3 r{x0 / mx, y0 / mx, 0.0};
+    const double ib = 1.0 / __builtin_sqrt(bsq);
+    const V3 b{B.x * ib, B.y * ib, B.z * ib};
+    const double rb = dot3(r, b);
+    r = V3{r.x - rb * b.x, r.y - rb * b.y, r.z - rb * b.z};
+    const double rn = 1.0 / __builtin_sqrt(dot3(r, r));
+    const V3 v1{r.x * rn, r.y * rn, r.z * rn};
+    const V3 v2 = cross3(b, v1);
+    double dI[2];
+    for (int k = 0; k < 2 && st == BST_OK; ++k) {
+      const V3 u = k == 0 ? v1 : v2;
+      double Ip, Imn;
+      st = line_integrals<M>(A, lane, t, x0 + L.dstep * u.x, y0 + L.dstep * u.y,
+                             z0 + L.dstep * u.z, Bm, nullptr, &Ip, nullptr);
+      if (st != BST_OK) break;
+      st = line_integrals<M>(A, lane, t, x0 - L.dstep * u.x, y0 - L.dstep * u.y,
+                             z0 - L.dstep * u.z, Bm, nullptr, &Imn, nullptr);
+      if (st != BST_OK) break;
+      if (Ip == 0.0) dI[k] = (ic - Imn) / L.dstep;
+      else if (Imn == 0.0) dI[k] = (Ip - ic) / L.dstep;
+      else dI[k] = (Ip - Imn) / (2.0 * L.dstep);
+    }
+    double Tb = __builtin_nan(""), wd = __builtin_nan(""), latd = __builtin_nan("");
+    if (st == BST_OK) {
+      const V3 gI{dI[0] * v1.x + dI[1] * v2.x, dI[0] * v1.y + dI[1] * v2.y,
+                  dI[0] * v1.z + dI[1] * v2.z};
+      const double coef = gamma * L.mass * v * v / (L.charge * sb * bsq);
+      const V3 c = cross3(gI, B);
+      const double rho = __builtin_sqrt(x0 * x0 + y0 * y0);
+      // phi_hat = (-y, x, 0) / rho
+      wd = coef * (c.y * x0 - c.x * y0) / (rho * rho);
+      Tb = 2.0 * sb / v;
+      latd = lat * (180.0 / 3.141592653589793);
+    }
+    A.out[3 * i] = Tb;
+    A.out[3 * i + 1] = wd;
+    A.out[3 * i + 2] = latd;
     A.status[i] = st;
   }
 }
@@ -321,33 +463,39 @@ static void launch_hb(rapt_ctx* c, const BounceArgs& A) {
                      c->stream, A);
 }
 
-extern "C" int rapt_halfbounce(rapt_ctx* c, int64_t n, const double* tpos, const double* Bm,
-                               double fieldlineresolution, double* sb, int32_t* status) {
-  if (!c || n < 0 || (n && (!tpos || !Bm || !sb || !status))) return RAPT_E_ARG;
-  if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
-  if (!(fieldlineresolution > 0.0))
-    return set_err(c, RAPT_E_ARG, "fieldlineresolution must be > 0");
-  if (n == 0) return RAPT_OK;
+template <int M>
+static void launch_lab(rapt_ctx* c, const BounceArgs& A, const LabelArgs& L) {
+  const int B = 256;
+  hipLaunchKernelGGL(k_bounce_labels<M>, dim3((unsigned)((A.lanes + B - 1) / B)), dim3(B), 0,
+                     c->stream, A, L);
+}
+
+// Shared host side: device copies of the inputs, the per-lane trace scratch,
+// one launch, results back.  in_w doubles in per query, out_w doubles out.
+static int run_bounce(rapt_ctx* c, int64_t n, const double* in, int in_w, const double* Bm,
+                      double res, bool positions, const LabelArgs* lab, double* out,
+                      int out_w, int32_t* status) {
   HIP_OK(c, hipSetDevice(c->device));
   BounceArgs A;
   memset(&A, 0, sizeof A);
   A.F = c->field;
   A.n = n;
-  A.res = fieldlineresolution;
+  A.res = res;
   A.cap = 2048;
   A.lanes = n < 16384 ? n : 16384;
   const size_t npts = (size_t)2 * A.cap * A.lanes;
-  double *d_tpos = nullptr, *d_bm = nullptr, *d_out = nullptr, *scratch = nullptr;
+  const int narr = positions ? 6 : 4;
+  double *d_in = nullptr, *d_bm = nullptr, *d_out = nullptr, *scratch = nullptr;
   int32_t* d_st = nullptr;
   int rc = RAPT_OK;
-  if (hipMalloc(&d_tpos, sizeof(double) * 4 * n) != hipSuccess ||
-      hipMalloc(&d_bm, sizeof(double) * n) != hipSuccess ||
-      hipMalloc(&d_out, sizeof(double) * n) != hipSuccess ||
+  if (hipMalloc(&d_in, sizeof(double) * in_w * n) != hipSuccess ||
+      (Bm && hipMalloc(&d_bm, sizeof(double) * n) != hipSuccess) ||
+      hipMalloc(&d_out, sizeof(double) * out_w * n) != hipSuccess ||
       hipMalloc(&d_st, sizeof(int32_t) * n) != hipSuccess ||
-      hipMalloc(&scratch, sizeof(double) * 4 * npts) != hipSuccess) {
+      hipMalloc(&scratch, sizeof(double) * narr * npts) != hipSuccess) {
     rc = set_err(c, RAPT_E_OOM, "bounce scratch allocation failed");
   } else {
-    A.tpos = d_tpos;
+    A.tpos = d_in;
     A.Bm = d_bm;
     A.out = d_out;
     A.status = d_st;
@@ -355,25 +503,27 @@ extern "C" int rapt_halfbounce(rapt_ctx* c, int64_t n, const double* tpos, const
     A.Bv = scratch + npts;
     A.Cc = scratch + 2 * npts;
     A.Wk = scratch + 3 * npts;
-    if (hipMemcpyAsync(d_tpos, tpos, sizeof(double) * 4 * n, hipMemcpyHostToDevice,
+    A.Zs = positions ? scratch + 4 * npts : nullptr;
+    A.Rs = positions ? scratch + 5 * npts : nullptr;
+    if (hipMemcpyAsync(d_in, in, sizeof(double) * in_w * n, hipMemcpyHostToDevice,
                        c->stream) != hipSuccess ||
-        hipMemcpyAsync(d_bm, Bm, sizeof(double) * n, hipMemcpyHostToDevice, c->stream) !=
-            hipSuccess) {
+        (Bm && hipMemcpyAsync(d_bm, Bm, sizeof(double) * n, hipMemcpyHostToDevice,
+                              c->stream) != hipSuccess)) {
       rc = set_err(c, RAPT_E_HIP, "bounce input copy failed");
     } else {
       switch (c->field.model) {
-        case 1: launch_hb<1>(c, A); break;
-        case 2: launch_hb<2>(c, A); break;
-        case 5: launch_hb<5>(c, A); break;
-        case 8: launch_hb<8>(c, A); break;
+        case 1: lab ? launch_lab<1>(c, A, *lab) : launch_hb<1>(c, A); break;
+        case 2: lab ? launch_lab<2>(c, A, *lab) : launch_hb<2>(c, A); break;
+        case 5: lab ? launch_lab<5>(c, A, *lab) : launch_hb<5>(c, A); break;
+        case 8: lab ? launch_lab<8>(c, A, *lab) : launch_hb<8>(c, A); break;
         default:
-          rc = set_err(c, RAPT_E_ARG, "bounce period needs a dipole-type field (model %d)",
+          rc = set_err(c, RAPT_E_ARG, "field-line integrals need a dipole-type field (model %d)",
                        c->field.model);
       }
       if (rc == RAPT_OK) {
         if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(sb, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) !=
-                hipSuccess ||
+            hipMemcpyAsync(out, d_out, sizeof(double) * out_w * n, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
             hipMemcpyAsync(status, d_st, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
                            c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
@@ -381,10 +531,35 @@ extern "C" int rapt_halfbounce(rapt_ctx* c, int64_t n, const double* tpos, const
       }
     }
   }
-  (void)hipFree(d_tpos);
+  (void)hipFree(d_in);
   (void)hipFree(d_bm);
   (void)hipFree(d_out);
   (void)hipFree(d_st);
   (void)hipFree(scratch);
   return rc;
+}
+
+extern "C" int rapt_halfbounce(rapt_ctx* c, int64_t n, const double* tpos, const double* Bm,
+                               double fieldlineresolution, double* sb, int32_t* status) {
+  if (!c || n < 0 || (n && (!tpos || !Bm || !sb || !status))) return RAPT_E_ARG;
+  if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
+  if (!(fieldlineresolution > 0.0))
+    return set_err(c, RAPT_E_ARG, "fieldlineresolution must be > 0");
+  if (n == 0) return RAPT_OK;
+  return run_bounce(c, n, tpos, 4, Bm, fieldlineresolution, false, nullptr, sb, 1, status);
+}
+
+extern "C" int rapt_bounce_labels(rapt_ctx* c, int64_t n, const double* query, double mass,
+                                  double charge, double Re, double eV, double eyegradientstep,
+                                  double fieldlineresolution, double* out, int32_t* status) {
+  if (!c || n < 0 || (n && (!query || !out || !status))) return RAPT_E_ARG;
+  if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
+  if (!(fieldlineresolution > 0.0) || !(eyegradientstep > 0.0) || !(mass > 0.0) ||
+      charge == 0.0 || !(Re > 0.0) || !(eV > 0.0))
+    return set_err(c, RAPT_E_ARG, "bad label parameters");
+  if (!c->field.is_static)
+    return set_err(c, RAPT_E_ARG, "bounce-averaged drift needs a static field");
+  if (n == 0) return RAPT_OK;
+  const LabelArgs L{mass, charge, eyegradientstep, Re, eV};
+  return run_bounce(c, n, query, 3, nullptr, fieldlineresolution, true, &L, out, 3, status);
 }

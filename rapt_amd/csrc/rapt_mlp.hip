@@ -1,0 +1,947 @@
+// rapt_mlp.hip -- the bounce-drift surrogate: an ensemble of MLPs
+// 3 -> 256 -> 256 -> 256 -> 3 with SiLU, BF16 weights and FP32 accumulation,
+// trained and evaluated by the kernels below on gfx950.  It has no
+// counterpart in the reference: it replaces the traced bounce quantities of
+// rapt_bounce_labels (BounceCenter's drift, rapt/BounceCenter.py:235-244,
+// and flutils.bounceperiod, rapt/flutils.py:232-252) where the ensemble
+// agrees, and the traced labels are the fallback where it does not.
+//
+// Model.  Query q = (L [Re], alpha_eq [deg], log10 K [eV]).  Features
+// x = ((L - 5)/3, (alpha - 49.5)/39.5, (log10 K - 5.5)/1.5).  Each output is a
+// log-residual against a dipole closed form b(q) (Walt's bounce and drift
+// period approximations, rapt/utils.py:435-513, and the exact dipole mirror
+// latitude): y = ln|label| - b(q), normalised by per-output shift/scale from
+// the training set.  The network predicts the normalised residual.
+//
+// Parameters per member, one FP32 master array theta (offsets below) with an
+// Adam state and a BF16 copy used by every forward pass.  Layer 1 (K = 3) and
+// the output layer (N = 3) run on the vector ALU in FP32 with BF16 weights;
+// the two 256 x 256 layers run on MFMA (v_mfma_f32_32x32x16_bf16) with BF16
+// activations.  The third hidden activation stays FP32 into the output layer.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <new>
+
+#include "../../include/rapt_hip.h"
+#include "rapt_internal.hpp"
+
+namespace rapt {
+namespace mlp {
+
+constexpr int H = 256;
+constexpr int OFF_W1 = 0;                 // [256][3]
+constexpr int OFF_B1 = OFF_W1 + H * 3;    // [256]
+constexpr int OFF_W2 = OFF_B1 + H;        // [256][256] (out, in)
+constexpr int OFF_B2 = OFF_W2 + H * H;
+constexpr int OFF_W3 = OFF_B2 + H;
+constexpr int OFF_B3 = OFF_W3 + H * H;
+constexpr int OFF_W4 = OFF_B3 + H;        // [3][256]
+constexpr int OFF_B4 = OFF_W4 + 3 * H;    // [3]
+constexpr int NPARAM = OFF_B4 + 3;        // 133,379
+constexpr int NPAD = (NPARAM + 255) / 256 * 256;
+constexpr int PACKED = 8 * 16 * 64 * 8;   // bf16 elements of one packed 256x256 layer
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// logistic function from v_exp_f32 and v_rcp_f32 (about 1 ulp each); training
+// and inference use this same definition
+__device__ __forceinline__ float sigm(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = sigm(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// ---- dipole baselines (the residual targets' reference point) -----------------------
+struct Base {
+  float lnT, lnW, lnLat;
+};
+// speedfromKE (rapt/utils.py:218-249) then Walt's approximations
+// (rapt/utils.py:435-513): T_b = 0.117 L / beta (1 - 0.4635 sin^0.75 a),
+// T_d = C / (L gamma beta^2) (1 - sin^0.62 a / 3) with C = 8.481 (protons,
+// scaled by m / m_p for other masses); the dipole mirror latitude solves
+// (1 - x)^3 = sin^2 a sqrt(1 + 3 x), x = sin^2 lambda, by Newton's method.
+__device__ __forceinline__ Base baseline(float L, float adeg, float lgk, double mass,
+                                         double eV) {
+  const double c = 299792458.0;
+  const double K = pow(10.0, (double)lgk) * eV;
+  const double mc2 = mass * c * c;
+  double beta, gam;
+  if (K / mc2 < 1e-6) {
+    beta = sqrt(2.0 * K / mass) / c;
+    gam = 1.0 / sqrt(1.0 - beta * beta);
+  } else {
+    gam = 1.0 + K / mc2;
+    beta = sqrt(1.0 - 1.0 / (gam * gam));
+  }
+  const double a = (double)adeg * (3.141592653589793 / 180.0);
+  const double sa = sin(a);
+  const double Tb = 0.117 * L / beta * (1.0 - 0.4635 * pow(sa, 0.75));
+  const double Cd = 8.481 * (mass / 1.672621777e-27);
+  const double Td = Cd / (L * gam * beta * beta) * (1.0 - pow(sa, 0.62) / 3.0);
+  const double s2 = sa * sa;
+  double x = 0.5 * (1.0 - s2);  // sin^2 lambda
+  for (int it = 0; it < 40; ++it) {
+    const double u = 1.0 - x, r = sqrt(1.0 + 3.0 * x);
+    const double g = u * u * u - s2 * r;
+    const double dg = -3.0 * u * u - 1.5 * s2 / r;
+    double xn = x - g / dg;
+    xn = fmin(fmax(xn, 0.0), 1.0);
+    if (fabs(xn - x) < 1e-15) { x = xn; break; }
+    x = xn;
+  }
+  const double lat = asin(sqrt(fmax(x, 1e-300))) * (180.0 / 3.141592653589793);
+  return Base{(float)log(Tb), (float)log(6.283185307179586 / Td), (float)log(lat)};
+}
+
+__device__ __forceinline__ void features(const float* q, float x[3]) {
+  x[0] = (q[0] - 5.0f) * (1.0f / 3.0f);
+  x[1] = (q[1] - 49.5f) * (1.0f / 39.5f);
+  x[2] = (q[2] - 5.5f) * (1.0f / 1.5f);
+}
+
+// ---- training kernels -----------------------------------------------------------------
+// Every kernel below runs all members at once: blockIdx.z (or y) = member,
+// with per-member strides.
+
+// C[M][N] = op(A)[M][K] * op(B)[K][N], BF16 in, FP32 out.  op(A)[m][k] =
+// A[m*lda+k] (TA false) or A[k*lda+m]; op(B)[k][n] = B[k*ldb+n] (TB false)
+// or B[n*ldb+k].  M, N multiples of 64, K a multiple of 16.  Block: 4 waves,
+// 64 x 64 tile, wave (w/2, w%2) owns a 32 x 32 quarter.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* A, int lda,
+                                              int64_t sA, const bf16* B, int ldb, int64_t sB,
+                                              float* C, int ldc, int64_t sC) {
+  __shared__ bf16 As[64][16 + 8];
+  __shared__ bf16 Bs[16][64 + 8];
+  const int mem = blockIdx.z;
+  A += mem * sA;
+  B += mem * sB;
+  C += mem * sC;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const int wm = w >> 1, wn = w & 1;
+  f32x16 acc = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    if (!TA) {
+      const int i = t >> 2, kk = (t & 3) * 4;
+      for (int e = 0; e < 4; ++e) As[i][kk + e] = A[(int64_t)(m0 + i) * lda + k0 + kk + e];
+    } else {
+      const int kk = t >> 4, i = (t & 15) * 4;
+      for (int e = 0; e < 4; ++e) As[i + e][kk] = A[(int64_t)(k0 + kk) * lda + m0 + i + e];
+    }
+    if (!TB) {
+      const int kk = t >> 4, j = (t & 15) * 4;
+      for (int e = 0; e < 4; ++e) Bs[kk][j + e] = B[(int64_t)(k0 + kk) * ldb + n0 + j + e];
+    } else {
+      const int j = t >> 2, kk = (t & 3) * 4;
+      for (int e = 0; e < 4; ++e) Bs[kk + e][j] = B[(int64_t)(n0 + j) * ldb + k0 + kk + e];
+    }
+    __syncthreads();
+    bf16x8 a, b;
+    for (int j = 0; j < 8; ++j) {
+      a[j] = As[wm * 32 + (l & 31)][8 * (l >> 5) + j];
+      b[j] = Bs[8 * (l >> 5) + j][wn * 32 + (l & 31)];
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    __syncthreads();
+  }
+  for (int r = 0; r < 16; ++r) {
+    const int row = wm * 32 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+    const int col = wn * 32 + (l & 31);
+    C[(int64_t)(m0 + row) * ldc + n0 + col] = acc[r];
+  }
+}
+
+// layer 1: Z1 = W1 x + b1, H1 = silu(Z1) (bf16).  grid (Bt*256/256, 1, members)
+__global__ void k_in_fwd(int Bt, const float* X, const bf16* wb, float* Z1, bf16* H1,
+                         int64_t sZ) {
+  const int mem = blockIdx.z;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)Bt * H) return;
+  const int s = (int)(idx / H), n = (int)(idx % H);
+  const bf16* w = wb + (int64_t)mem * NPAD;
+  float z = (float)w[OFF_B1 + n];
+  for (int i = 0; i < 3; ++i) z += (float)w[OFF_W1 + n * 3 + i] * X[s * 3 + i];
+  Z1[mem * sZ + idx] = z;
+  H1[mem * sZ + idx] = (bf16)silu(z);
+}
+
+// Z += b (layer bias at boff), H = silu(Z), as bf16 (Hb) or fp32 (Hf)
+__global__ void k_bias_act(int Bt, const bf16* wb, int boff, float* Z, bf16* Hb, float* Hf,
+                           int64_t sZ) {
+  const int mem = blockIdx.z;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)Bt * H) return;
+  const int n = (int)(idx % H);
+  const float z = Z[mem * sZ + idx] + (float)wb[(int64_t)mem * NPAD + boff + n];
+  Z[mem * sZ + idx] = z;
+  if (Hb) Hb[mem * sZ + idx] = (bf16)silu(z);
+  if (Hf) Hf[mem * sZ + idx] = silu(z);
+}
+
+// output layer and loss: Y = W4 H3 + b4; dY = 2 (Y - T) / (3 Bt); loss sum
+__global__ void k_out_fwd(int Bt, const float* H3, const bf16* wb, const float* T, float* Y,
+                          float* dY, double* loss, int64_t sZ) {
+  const int mem = blockIdx.z;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double part = 0.0;
+  if (s < Bt) {
+    const bf16* w = wb + (int64_t)mem * NPAD;
+    const float* h = H3 + mem * sZ + (int64_t)s * H;
+    for (int k = 0; k < 3; ++k) {
+      float y = (float)w[OFF_B4 + k];
+      for (int n = 0; n < H; ++n) y += (float)w[OFF_W4 + k * H + n] * h[n];
+      const int64_t o = (int64_t)mem * Bt * 3 + s * 3 + k;
+      Y[o] = y;
+      if (T) {
+        const float d = y - T[s * 3 + k];
+        dY[o] = 2.0f * d / (3.0f * Bt);
+        part += (double)d * d;
+      }
+    }
+  }
+  if (T && loss) {
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_down(part, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&loss[mem], part);
+  }
+}
+
+// dH3 = dY W4, dZ3 = dH3 silu'(Z3) -> bf16
+__global__ void k_out_bwd(int Bt, const float* dY, const bf16* wb, const float* Z3, bf16* dZ3,
+                          int64_t sZ) {
+  const int mem = blockIdx.z;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)Bt * H) return;
+  const int s = (int)(idx / H), n = (int)(idx % H);
+  const bf16* w = wb + (int64_t)mem * NPAD;
+  const float* dy = dY + (int64_t)mem * Bt * 3 + s * 3;
+  const float g = dy[0] * (float)w[OFF_W4 + n] + dy[1] * (float)w[OFF_W4 + H + n] +
+                  dy[2] * (float)w[OFF_W4 + 2 * H + n];
+  dZ3[mem * sZ + idx] = (bf16)(g * dsilu(Z3[mem * sZ + idx]));
+}
+
+// dZ = dH silu'(Z) -> bf16
+__global__ void k_act_bwd(int Bt, const float* dH, const float* Z, bf16* dZ, int64_t sZ) {
+  const int mem = blockIdx.z;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)Bt * H) return;
+  dZ[mem * sZ + idx] = (bf16)(dH[mem * sZ + idx] * dsilu(Z[mem * sZ + idx]));
+}
+
+// column sums over the batch: g[goff + n] = sum_s dZ[s][n] (bias gradients)
+__global__ void k_colsum(int Bt, const bf16* dZ, float* grad, int goff, int64_t sZ) {
+  const int mem = blockIdx.z;
+  const int n = threadIdx.x;
+  float acc = 0.0f;
+  const bf16* p = dZ + mem * sZ;
+  for (int s = 0; s < Bt; ++s) acc += (float)p[(int64_t)s * H + n];
+  grad[(int64_t)mem * NPAD + goff + n] = acc;
+}
+
+// small reductions: dW4[k][n] = sum_s dY[s][k] H3[s][n], db4, dW1[n][i] =
+// sum_s dZ1[s][n] X[s][i] (dZ1 = dH1 silu'(Z1) formed here in fp32)
+__global__ void k_small_grads(int Bt, const float* dY, const float* H3, const float* dH1,
+                              const float* Z1, const float* X, float* grad, int64_t sZ) {
+  const int mem = blockIdx.z;
+  const int n = threadIdx.x;  // 256 threads
+  const float* h3 = H3 + mem * sZ;
+  const float* dh1 = dH1 + mem * sZ;
+  const float* z1 = Z1 + mem * sZ;
+  const float* dy = dY + (int64_t)mem * Bt * 3;
+  float w4[3] = {0, 0, 0}, w1[3] = {0, 0, 0}, b1 = 0.0f, b4 = 0.0f;
+  for (int s = 0; s < Bt; ++s) {
+    const float hv = h3[(int64_t)s * H + n];
+    for (int k = 0; k < 3; ++k) w4[k] += dy[s * 3 + k] * hv;
+    const float dz = dh1[(int64_t)s * H + n] * dsilu(z1[(int64_t)s * H + n]);
+    b1 += dz;
+    for (int i = 0; i < 3; ++i) w1[i] += dz * X[s * 3 + i];
+    if (n < 3) b4 += dy[s * 3 + n];
+  }
+  float* g = grad + (int64_t)mem * NPAD;
+  for (int k = 0; k < 3; ++k) g[OFF_W4 + k * H + n] = w4[k];
+  for (int i = 0; i < 3; ++i) g[OFF_W1 + n * 3 + i] = w1[i];
+  g[OFF_B1 + n] = b1;
+  if (n < 3) g[OFF_B4 + n] = b4;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 masters; refreshes the BF16 copy
+__global__ void k_adam(float* theta, const float* grad, float* m1, float* m2, bf16* wb,
+                       float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int mem = blockIdx.z;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NPARAM) return;
+  const int64_t o = (int64_t)mem * NPAD + i;
+  const float g = grad[o];
+  const float a = b1 * m1[o] + (1.0f - b1) * g;
+  const float v = b2 * m2[o] + (1.0f - b2) * g * g;
+  m1[o] = a;
+  m2[o] = v;
+  const float p = theta[o] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+  theta[o] = p;
+  wb[o] = (bf16)p;
+}
+
+__global__ void k_gather(int bt, const int64_t* ord, const float* Xa, const float* Ta, float* Xb,
+                         float* Tb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= bt) return;
+  const int64_t j = ord[i];
+  for (int k = 0; k < 3; ++k) {
+    Xb[3 * i + k] = Xa[3 * j + k];
+    Tb[3 * i + k] = Ta[3 * j + k];
+  }
+}
+
+__global__ void k_zero_d(double* p, int k) {
+  if ((int)threadIdx.x < k) p[threadIdx.x] = 0.0;
+}
+
+__global__ void k_zero_f(float* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+__global__ void k_to_bf16(const float* theta, bf16* wb, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) wb[i] = (bf16)theta[i];
+}
+
+// ---- fused inference -------------------------------------------------------------------
+// Packed weights of a 256x256 layer for the register-resident forward pass.
+// The layer output of tile t (neurons 32t..32t+31) leaves lane l, register r
+// holding neuron 32t + 8(r/4) + 4(l/32) + r%4 (the MFMA D layout); the next
+// layer consumes registers 8u..8u+7 of tile t as its K-block 2t+u, so K index
+// 8h + j of block kb (h = lane half) is input neuron
+//   pi(kb, h, j) = 32(kb/2) + 16(kb%2) + 8(j/4) + 4h + j%4.
+// Packed[t'][kb][lane][j] = W[32t' + lane%32][pi(kb, lane/32, j)].
+__device__ __forceinline__ int pin(int kb, int h, int j) {
+  return 32 * (kb >> 1) + 16 * (kb & 1) + 8 * (j >> 2) + 4 * h + (j & 3);
+}
+__global__ void k_pack(const bf16* wb, bf16* packed, int woff) {
+  const int mem = blockIdx.z;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // over 8*16*64*8
+  if (idx >= PACKED) return;
+  const int j = idx & 7, lane = (idx >> 3) & 63, kb = (idx >> 9) & 15, tp = idx >> 13;
+  const int row = 32 * tp + (lane & 31), col = pin(kb, lane >> 5, j);
+  packed[(int64_t)mem * 2 * PACKED + idx] = wb[(int64_t)mem * NPAD + woff + row * H + col];
+}
+
+// Layer 1 as an MFMA: the three features are split into bf16 hi and lo parts
+// (x = hi + lo to 2^-17 relative), so K index j < 3 multiplies W1[.][j] by
+// hi_j and 3 <= j < 6 multiplies W1[.][j-3] by lo_{j-3}: the product with the
+// bf16 weights is as accurate as the FP32 vector form.  P1[t][lane][j] =
+// A-operand of output tile t.
+__global__ void k_pack1(const bf16* wb, bf16* p1) {
+  const int mem = blockIdx.z;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // 8 tiles * 64 lanes * 8
+  if (idx >= 8 * 64 * 8) return;
+  const int j = idx & 7, lane = (idx >> 3) & 63, t = idx >> 9;
+  const int n = 32 * t + (lane & 31);
+  bf16 v = (bf16)0.0f;
+  if ((lane >> 5) == 0 && j < 6) v = wb[(int64_t)mem * NPAD + OFF_W1 + n * 3 + (j % 3)];
+  p1[(int64_t)mem * 8 * 64 * 8 + idx] = v;
+}
+
+// per-member small parameters staged in LDS: (b1), (b2), (b3, w4[0..2]), b4
+struct SmallW {
+  float b1[H], b2[H];
+  float4 l3[H];
+  float b4[4];
+};
+constexpr int MAXM = 4;  // largest ensemble (all members' small parameters sit in LDS)
+
+// one member's forward pass for the wave's 32 queries (lane half h = l/32
+// holds half of each activation); returns the three normalised outputs in
+// every lane of the query.
+__device__ __forceinline__ void member_forward(const SmallW& sw, const bf16x8* P1,
+                                               const bf16x8* P2, const bf16x8* P3,
+                                               const bf16x8& xb, int l, float out[3]) {
+  const int h = l >> 5;
+  bf16x8 act[16];
+  // layer 1: D layout of tile t feeds K-blocks 2t, 2t+1 of layer 2
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    f32x16 acc = {};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P1[t * 64 + l], xb, acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
+      const bf16 v = (bf16)silu(acc[r] + sw.b1[n]);
+      if (r < 8) act[2 * t][r] = v; else act[2 * t + 1][r - 8] = v;
+    }
+  }
+  // layer 2 (fully unrolled: nxt is indexed by the tile)
+  bf16x8 nxt[16];
+#pragma unroll
+  for (int tp = 0; tp < 8; ++tp) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int kb = 0; kb < 16; ++kb)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P2[(tp * 16 + kb) * 64 + l], act[kb], acc,
+                                                     0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n = 32 * tp + 8 * (r >> 2) + 4 * h + (r & 3);
+      const bf16 v = (bf16)silu(acc[r] + sw.b2[n]);
+      if (r < 8) nxt[2 * tp][r] = v; else nxt[2 * tp + 1][r - 8] = v;
+    }
+  }
+  // layer 3 and the output layer (FP32 activations)
+  float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+#pragma unroll 1
+  for (int tp = 0; tp < 8; ++tp) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int kb = 0; kb < 16; ++kb)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P3[(tp * 16 + kb) * 64 + l], nxt[kb], acc,
+                                                     0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n = 32 * tp + 8 * (r >> 2) + 4 * h + (r & 3);
+      const float4 c = sw.l3[n];
+      const float v = silu(acc[r] + c.x);
+      o0 += c.y * v;
+      o1 += c.z * v;
+      o2 += c.w * v;
+    }
+  }
+  o0 += __shfl_xor(o0, 32);
+  o1 += __shfl_xor(o1, 32);
+  o2 += __shfl_xor(o2, 32);
+  out[0] = o0 + sw.b4[0];
+  out[1] = o1 + sw.b4[1];
+  out[2] = o2 + sw.b4[2];
+}
+
+struct InferArgs {
+  int64_t n;
+  int members;
+  const float* query;     // [n][3]
+  const bf16* wb;         // [members][NPAD]
+  const bf16* packed;     // [members][2][PACKED]
+  const bf16* p1;         // [members][8*64*8]
+  const float* norm;      // shift[3], scale[3]
+  double mass, eV;
+  float thresh;           // relative member deviation that sends a query to the fallback
+  float* out;             // [n][3] physical outputs (T_b, |omega_d|, latitude)
+  float* raw;             // optional [members][n][3] normalised outputs
+  int32_t* flag;          // [n] 1 = fallback needed
+};
+
+// Persistent over 32-query groups: each wave takes groups g = wave, wave +
+// waves, ...; the block stages every member's small parameters once.  Two
+// waves per SIMD (256 registers, a few spilled) beat one wave without spills
+// by 1.9x and three or four waves with heavier spilling (measured on the
+// 2^24-query ensemble inference).
+#ifndef RAPT_MLP_WAVES
+#define RAPT_MLP_WAVES 2
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAPT_MLP_WAVES)))
+void k_infer(InferArgs A) {
+  __shared__ SmallW sw[MAXM];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  const int64_t groups = (A.n + 31) / 32;
+  for (int k = 0; k < A.members; ++k) {
+    const bf16* w = A.wb + (int64_t)k * NPAD;
+    for (int i = threadIdx.x; i < H; i += 256) {
+      sw[k].b1[i] = (float)w[OFF_B1 + i];
+      sw[k].b2[i] = (float)w[OFF_B2 + i];
+      sw[k].l3[i] = make_float4((float)w[OFF_B3 + i], (float)w[OFF_W4 + i],
+                                (float)w[OFF_W4 + H + i], (float)w[OFF_W4 + 2 * H + i]);
+    }
+    if (threadIdx.x < 3) sw[k].b4[threadIdx.x] = (float)w[OFF_B4 + threadIdx.x];
+  }
+  __syncthreads();
+  for (int64_t g = (int64_t)blockIdx.x * 4 + wv; g < groups; g += waves) {
+    const int64_t qi = g * 32 + (l & 31);
+    const bool valid = qi < A.n;
+    float q[3], xin[3];
+    q[0] = valid ? A.query[qi * 3] : 5.0f;
+    q[1] = valid ? A.query[qi * 3 + 1] : 45.0f;
+    q[2] = valid ? A.query[qi * 3 + 2] : 5.0f;
+    features(q, xin);
+    bf16x8 xb;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xb[j] = (bf16)0.0f;
+    if (l < 32) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const bf16 hi = (bf16)xin[j];
+        xb[j] = hi;
+        xb[3 + j] = (bf16)(xin[j] - (float)hi);
+      }
+    }
+    float mean[3] = {0.0f, 0.0f, 0.0f}, lo[3], hi[3];
+    for (int c = 0; c < 3; ++c) { lo[c] = 3.0e38f; hi[c] = -3.0e38f; }
+    for (int m = 0; m < A.members; ++m) {
+      const bf16x8* P1 = (const bf16x8*)(A.p1 + (int64_t)m * 8 * 64 * 8);
+      const bf16x8* P2 = (const bf16x8*)(A.packed + ((int64_t)m * 2) * PACKED);
+      const bf16x8* P3 = (const bf16x8*)(A.packed + ((int64_t)m * 2 + 1) * PACKED);
+      float o[3];
+      member_forward(sw[m], P1, P2, P3, xb, l, o);
+      for (int c = 0; c < 3; ++c) {
+        mean[c] += o[c];
+        lo[c] = fminf(lo[c], o[c]);
+        hi[c] = fmaxf(hi[c], o[c]);
+      }
+      if (A.raw && valid && l < 32)
+        for (int c = 0; c < 3; ++c) A.raw[((int64_t)m * A.n + qi) * 3 + c] = o[c];
+    }
+    if (valid && l < 32) {
+      const Base b = baseline(q[0], q[1], q[2], A.mass, A.eV);
+      const float bb[3] = {b.lnT, b.lnW, b.lnLat};
+      int bad = 0;
+      for (int k = 0; k < 3; ++k) {
+        const float sc = A.norm[3 + k];
+        const float mu = mean[k] / (float)A.members;
+        // largest relative deviation of a member from the ensemble mean:
+        // exp(sc * max(hi - mu, mu - lo)) - 1
+        if (expm1f(sc * fmaxf(hi[k] - mu, mu - lo[k])) > A.thresh) bad = 1;
+        A.out[qi * 3 + k] = expf(bb[k] + A.norm[k] + sc * mu);
+      }
+      A.flag[qi] = bad;
+    }
+  }
+}
+
+// targets: normalised log-residuals from traced labels (T_b, omega_d, lat)
+__global__ void k_targets(int64_t n, const float* query, const double* labels,
+                          const float* norm, double mass, double eV, float* X, float* T) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x[3];
+  features(query + 3 * i, x);
+  const Base b = baseline(query[3 * i], query[3 * i + 1], query[3 * i + 2], mass, eV);
+  const float bb[3] = {b.lnT, b.lnW, b.lnLat};
+  for (int k = 0; k < 3; ++k) {
+    X[3 * i + k] = x[k];
+    const double lab = fabs(labels[3 * i + k]);
+    T[3 * i + k] = norm ? ((float)(log(lab) - (double)bb[k]) - norm[k]) / norm[3 + k]
+                        : (float)(log(lab) - (double)bb[k]);
+  }
+}
+
+}  // namespace mlp
+}  // namespace rapt
+
+using namespace rapt;
+using namespace rapt::mlp;
+
+struct rapt_mlp {
+  rapt_ctx* ctx = nullptr;
+  int members = 0;
+  int batch = 0;
+  double mass = 0.0, eV = 0.0;
+  int64_t step = 0;
+  float norm_h[6] = {0, 0, 0, 1, 1, 1};
+  float* theta = nullptr;
+  float* grad = nullptr;
+  float* m1 = nullptr;
+  float* m2 = nullptr;
+  bf16* wb = nullptr;
+  bf16* packed = nullptr;
+  bf16* p1 = nullptr;
+  float* norm = nullptr;
+  bool packed_ok = false;
+  // training workspace (batch-sized)
+  float *X = nullptr, *T = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr;
+  float *H3 = nullptr, *dH = nullptr, *Y = nullptr, *dY = nullptr;
+  bf16 *H1 = nullptr, *H2 = nullptr, *dZb = nullptr;
+  double* loss = nullptr;
+};
+
+static void mlp_free_work(rapt_mlp* m) {
+  void* p[] = {m->X, m->T, m->Z1, m->Z2, m->Z3, m->H3, m->dH, m->Y, m->dY,
+               m->H1, m->H2, m->dZb, m->loss};
+  for (void* q : p) (void)hipFree(q);
+  m->X = m->T = m->Z1 = m->Z2 = m->Z3 = m->H3 = m->dH = m->Y = m->dY = nullptr;
+  m->H1 = m->H2 = m->dZb = nullptr;
+  m->loss = nullptr;
+  m->batch = 0;
+}
+
+static int mlp_work(rapt_mlp* m, int bt) {
+  if (m->batch == bt) return RAPT_OK;
+  mlp_free_work(m);
+  rapt_ctx* c = m->ctx;
+  const size_t act = (size_t)m->members * bt * H;
+  bool ok = hipMalloc(&m->X, sizeof(float) * 3 * bt) == hipSuccess &&
+            hipMalloc(&m->T, sizeof(float) * 3 * bt) == hipSuccess &&
+            hipMalloc(&m->Z1, sizeof(float) * act) == hipSuccess &&
+            hipMalloc(&m->Z2, sizeof(float) * act) == hipSuccess &&
+            hipMalloc(&m->Z3, sizeof(float) * act) == hipSuccess &&
+            hipMalloc(&m->H3, sizeof(float) * act) == hipSuccess &&
+            hipMalloc(&m->dH, sizeof(float) * act) == hipSuccess &&
+            hipMalloc(&m->Y, sizeof(float) * 3 * bt * m->members) == hipSuccess &&
+            hipMalloc(&m->dY, sizeof(float) * 3 * bt * m->members) == hipSuccess &&
+            hipMalloc(&m->H1, sizeof(bf16) * act) == hipSuccess &&
+            hipMalloc(&m->H2, sizeof(bf16) * act) == hipSuccess &&
+            hipMalloc(&m->dZb, sizeof(bf16) * act) == hipSuccess &&
+            hipMalloc(&m->loss, sizeof(double) * m->members) == hipSuccess;
+  if (!ok) {
+    mlp_free_work(m);
+    return set_err(c, RAPT_E_OOM, "MLP workspace allocation failed");
+  }
+  m->batch = bt;
+  return RAPT_OK;
+}
+
+// forward of a training batch already in m->X; T may be null (no loss)
+static void mlp_forward(rapt_mlp* m, int bt, bool with_loss) {
+  hipStream_t st = m->ctx->stream;
+  const int mem = m->members;
+  const int64_t sZ = (int64_t)bt * H;
+  const dim3 ew((unsigned)((sZ + 255) / 256), 1, mem);
+  const dim3 gg(H / 64, bt / 64, mem);
+  hipLaunchKernelGGL(k_in_fwd, ew, dim3(256), 0, st, bt, m->X, m->wb, m->Z1, m->H1, sZ);
+  // Z2 = H1 W2^T
+  hipLaunchKernelGGL((k_gemm<false, true>), gg, dim3(256), 0, st, bt, H, H, m->H1, H, sZ,
+                     m->wb + OFF_W2, H, (int64_t)NPAD, m->Z2, H, sZ);
+  hipLaunchKernelGGL(k_bias_act, ew, dim3(256), 0, st, bt, m->wb, OFF_B2, m->Z2, m->H2,
+                     (float*)nullptr, sZ);
+  hipLaunchKernelGGL((k_gemm<false, true>), gg, dim3(256), 0, st, bt, H, H, m->H2, H, sZ,
+                     m->wb + OFF_W3, H, (int64_t)NPAD, m->Z3, H, sZ);
+  hipLaunchKernelGGL(k_bias_act, ew, dim3(256), 0, st, bt, m->wb, OFF_B3, m->Z3,
+                     (bf16*)nullptr, m->H3, sZ);
+  hipLaunchKernelGGL(k_out_fwd, dim3((unsigned)((bt + 255) / 256), 1, mem), dim3(256), 0, st,
+                     bt, m->H3, m->wb, with_loss ? m->T : (const float*)nullptr, m->Y, m->dY,
+                     m->loss, sZ);
+}
+
+static void mlp_backward(rapt_mlp* m, int bt) {
+  hipStream_t st = m->ctx->stream;
+  const int mem = m->members;
+  const int64_t sZ = (int64_t)bt * H;
+  const dim3 ew((unsigned)((sZ + 255) / 256), 1, mem);
+  const dim3 gw(H / 64, H / 64, mem);   // 256 x 256 weight gradients
+  const dim3 ga(H / 64, bt / 64, mem);  // batch x 256 activation gradients
+  // layer 3
+  hipLaunchKernelGGL(k_out_bwd, ew, dim3(256), 0, st, bt, m->dY, m->wb, m->Z3, m->dZb, sZ);
+  hipLaunchKernelGGL((k_gemm<true, false>), gw, dim3(256), 0, st, H, H, bt, m->dZb, H, sZ,
+                     m->H2, H, sZ, m->grad + OFF_W3, H, (int64_t)NPAD);
+  hipLaunchKernelGGL(k_colsum, dim3(1, 1, mem), dim3(256), 0, st, bt, m->dZb, m->grad, OFF_B3,
+                     sZ);
+  hipLaunchKernelGGL((k_gemm<false, false>), ga, dim3(256), 0, st, bt, H, H, m->dZb, H, sZ,
+                     m->wb + OFF_W3, H, (int64_t)NPAD, m->dH, H, sZ);
+  // layer 2
+  hipLaunchKernelGGL(k_act_bwd, ew, dim3(256), 0, st, bt, m->dH, m->Z2, m->dZb, sZ);
+  hipLaunchKernelGGL((k_gemm<true, false>), gw, dim3(256), 0, st, H, H, bt, m->dZb, H, sZ,
+                     m->H1, H, sZ, m->grad + OFF_W2, H, (int64_t)NPAD);
+  hipLaunchKernelGGL(k_colsum, dim3(1, 1, mem), dim3(256), 0, st, bt, m->dZb, m->grad, OFF_B2,
+                     sZ);
+  hipLaunchKernelGGL((k_gemm<false, false>), ga, dim3(256), 0, st, bt, H, H, m->dZb, H, sZ,
+                     m->wb + OFF_W2, H, (int64_t)NPAD, m->dH, H, sZ);
+  // layer 1 and the output layer
+  hipLaunchKernelGGL(k_small_grads, dim3(1, 1, mem), dim3(256), 0, st, bt, m->dY, m->H3, m->dH,
+                     m->Z1, m->X, m->grad, sZ);
+}
+
+extern "C" {
+
+int rapt_mlp_create(rapt_ctx* c, int32_t members, const float* theta0, double mass, double eV,
+                    rapt_mlp** out) {
+  if (!c || !out || members < 1 || members > MAXM || !theta0 || !(mass > 0.0) || !(eV > 0.0))
+    return c ? set_err(c, RAPT_E_ARG, "bad MLP arguments") : RAPT_E_ARG;
+  HIP_OK(c, hipSetDevice(c->device));
+  rapt_mlp* m = new (std::nothrow) rapt_mlp();
+  if (!m) return RAPT_E_OOM;
+  m->ctx = c;
+  m->members = members;
+  m->mass = mass;
+  m->eV = eV;
+  const size_t np = (size_t)members * NPAD;
+  bool ok = hipMalloc(&m->theta, sizeof(float) * np) == hipSuccess &&
+            hipMalloc(&m->grad, sizeof(float) * np) == hipSuccess &&
+            hipMalloc(&m->m1, sizeof(float) * np) == hipSuccess &&
+            hipMalloc(&m->m2, sizeof(float) * np) == hipSuccess &&
+            hipMalloc(&m->wb, sizeof(bf16) * np) == hipSuccess &&
+            hipMalloc(&m->packed, sizeof(bf16) * members * 2 * PACKED) == hipSuccess &&
+            hipMalloc(&m->p1, sizeof(bf16) * members * 8 * 64 * 8) == hipSuccess &&
+            hipMalloc(&m->norm, sizeof(float) * 6) == hipSuccess;
+  if (!ok) {
+    rapt_mlp_destroy(m);
+    return set_err(c, RAPT_E_OOM, "MLP allocation failed");
+  }
+  hipStream_t st = c->stream;
+  hipLaunchKernelGGL(k_zero_f, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, m->theta,
+                     (int64_t)np);
+  for (int k = 0; k < members; ++k)
+    if (hipMemcpyAsync(m->theta + (size_t)k * NPAD, theta0 + (size_t)k * NPARAM,
+                       sizeof(float) * NPARAM, hipMemcpyHostToDevice, st) != hipSuccess) {
+      rapt_mlp_destroy(m);
+      return set_err(c, RAPT_E_HIP, "MLP upload failed");
+    }
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, m->theta,
+                     m->wb, (int64_t)np);
+  hipLaunchKernelGGL(k_zero_f, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, m->m1,
+                     (int64_t)np);
+  hipLaunchKernelGGL(k_zero_f, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, m->m2,
+                     (int64_t)np);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(m->norm, m->norm_h, sizeof(float) * 6, hipMemcpyHostToDevice, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    rapt_mlp_destroy(m);
+    return set_err(c, RAPT_E_HIP, "MLP initialisation failed");
+  }
+  *out = m;
+  return RAPT_OK;
+}
+
+int rapt_mlp_destroy(rapt_mlp* m) {
+  if (!m) return RAPT_E_ARG;
+  if (m->ctx) (void)hipSetDevice(m->ctx->device);
+  mlp_free_work(m);
+  void* p[] = {m->theta, m->grad, m->m1, m->m2, m->wb, m->packed, m->p1, m->norm};
+  for (void* q : p) (void)hipFree(q);
+  delete m;
+  return RAPT_OK;
+}
+
+int rapt_mlp_set_norm(rapt_mlp* m, const float* shift_scale) {
+  if (!m || !shift_scale) return RAPT_E_ARG;
+  rapt_ctx* c = m->ctx;
+  for (int k = 0; k < 3; ++k)
+    if (!(shift_scale[3 + k] > 0.0f)) return set_err(c, RAPT_E_ARG, "scales must be > 0");
+  memcpy(m->norm_h, shift_scale, sizeof m->norm_h);
+  HIP_OK(c, hipMemcpyAsync(m->norm, m->norm_h, sizeof(float) * 6, hipMemcpyHostToDevice,
+                           c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  return RAPT_OK;
+}
+
+int rapt_mlp_targets(rapt_mlp* m, int64_t n, const float* query, const double* labels,
+                     int normalised, float* X, float* T) {
+  if (!m || n <= 0 || !query || !labels || !X || !T) return m ? set_err(m->ctx, RAPT_E_ARG,
+                                                                        "bad targets call")
+                                                              : RAPT_E_ARG;
+  rapt_ctx* c = m->ctx;
+  HIP_OK(c, hipSetDevice(c->device));
+  float *dq = nullptr, *dX = nullptr, *dT = nullptr;
+  double* dl = nullptr;
+  int rc = RAPT_OK;
+  if (hipMalloc(&dq, sizeof(float) * 3 * n) != hipSuccess ||
+      hipMalloc(&dl, sizeof(double) * 3 * n) != hipSuccess ||
+      hipMalloc(&dX, sizeof(float) * 3 * n) != hipSuccess ||
+      hipMalloc(&dT, sizeof(float) * 3 * n) != hipSuccess) {
+    rc = set_err(c, RAPT_E_OOM, "targets allocation failed");
+  } else if (hipMemcpyAsync(dq, query, sizeof(float) * 3 * n, hipMemcpyHostToDevice,
+                            c->stream) != hipSuccess ||
+             hipMemcpyAsync(dl, labels, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                            c->stream) != hipSuccess) {
+    rc = set_err(c, RAPT_E_HIP, "targets copy failed");
+  } else {
+    hipLaunchKernelGGL(k_targets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       n, dq, dl, normalised ? m->norm : (const float*)nullptr, m->mass, m->eV,
+                       dX, dT);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(X, dX, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(T, dT, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = set_err(c, RAPT_E_HIP, "targets kernel failed");
+  }
+  (void)hipFree(dq);
+  (void)hipFree(dl);
+  (void)hipFree(dX);
+  (void)hipFree(dT);
+  return rc;
+}
+
+int rapt_mlp_train(rapt_mlp* m, int64_t n, const float* X, const float* T, int32_t batch,
+                   int32_t steps, const int64_t* order, float lr0, float lr1, double* loss_out,
+                   float* grad_out) {
+  if (!m) return RAPT_E_ARG;
+  rapt_ctx* c = m->ctx;
+  if (n <= 0 || !X || !T || batch < 64 || batch % 64 != 0 || steps < 0 || !order)
+    return set_err(c, RAPT_E_ARG, "train: batch must be a positive multiple of 64");
+  HIP_OK(c, hipSetDevice(c->device));
+  int rc = mlp_work(m, batch);
+  if (rc) return rc;
+  hipStream_t st = c->stream;
+  float *dXall = nullptr, *dTall = nullptr;
+  int64_t* dorder = nullptr;
+  const int64_t nidx = (int64_t)steps * batch;
+  if (hipMalloc(&dXall, sizeof(float) * 3 * n) != hipSuccess ||
+      hipMalloc(&dTall, sizeof(float) * 3 * n) != hipSuccess ||
+      hipMalloc(&dorder, sizeof(int64_t) * (nidx > 0 ? nidx : 1)) != hipSuccess) {
+    (void)hipFree(dXall);
+    (void)hipFree(dTall);
+    (void)hipFree(dorder);
+    return set_err(c, RAPT_E_OOM, "training data allocation failed");
+  }
+  for (int64_t i = 0; i < nidx; ++i)
+    if (order[i] < 0 || order[i] >= n) {
+      (void)hipFree(dXall);
+      (void)hipFree(dTall);
+      (void)hipFree(dorder);
+      return set_err(c, RAPT_E_ARG, "sample index out of range");
+    }
+  rc = RAPT_OK;
+  if (hipMemcpyAsync(dXall, X, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(dTall, T, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st) != hipSuccess ||
+      (nidx > 0 && hipMemcpyAsync(dorder, order, sizeof(int64_t) * nidx, hipMemcpyHostToDevice,
+                                  st) != hipSuccess))
+    rc = set_err(c, RAPT_E_HIP, "training data copy failed");
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int s = 0; s < steps && rc == RAPT_OK; ++s) {
+    // gather the batch
+    hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, st, batch,
+                       dorder + (int64_t)s * batch, dXall, dTall, m->X, m->T);
+    const bool last = s == steps - 1;
+    if (last && loss_out)
+      hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, st, m->loss, m->members);
+    mlp_forward(m, batch, true);
+    mlp_backward(m, batch);
+    if (last && grad_out)
+      for (int k = 0; k < m->members; ++k)
+        if (hipMemcpyAsync(grad_out + (size_t)k * NPARAM, m->grad + (size_t)k * NPAD,
+                           sizeof(float) * NPARAM, hipMemcpyDeviceToHost, st) != hipSuccess)
+          rc = set_err(c, RAPT_E_HIP, "gradient copy failed");
+    // cosine learning-rate schedule from lr0 to lr1 over this call
+    const float frac = steps > 1 ? (float)s / (float)(steps - 1) : 0.0f;
+    const float lr = lr1 + 0.5f * (lr0 - lr1) * (1.0f + cosf(3.14159265f * frac));
+    ++m->step;
+    const float c1 = 1.0f - powf(b1, (float)m->step), c2 = 1.0f - powf(b2, (float)m->step);
+    if (lr0 > 0.0f)
+      hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256, 1, m->members), dim3(256), 0, st,
+                         m->theta, m->grad, m->m1, m->m2, m->wb, lr, b1, b2, eps, c1, c2);
+  }
+  if (rc == RAPT_OK && loss_out) {
+    double lh[16];
+    if (hipMemcpyAsync(lh, m->loss, sizeof(double) * m->members, hipMemcpyDeviceToHost, st) !=
+        hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+      rc = set_err(c, RAPT_E_HIP, "loss copy failed");
+    for (int k = 0; k < m->members && rc == RAPT_OK; ++k) loss_out[k] = lh[k] / (3.0 * batch);
+  }
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+    rc = rc ? rc : set_err(c, RAPT_E_HIP, "training kernels failed");
+  m->packed_ok = false;
+  (void)hipFree(dXall);
+  (void)hipFree(dTall);
+  (void)hipFree(dorder);
+  return rc;
+}
+
+int rapt_mlp_get_params(rapt_mlp* m, float* theta) {
+  if (!m || !theta) return RAPT_E_ARG;
+  rapt_ctx* c = m->ctx;
+  HIP_OK(c, hipSetDevice(c->device));
+  for (int k = 0; k < m->members; ++k)
+    HIP_OK(c, hipMemcpyAsync(theta + (size_t)k * NPARAM, m->theta + (size_t)k * NPAD,
+                             sizeof(float) * NPARAM, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  return RAPT_OK;
+}
+
+// Forward pass through the training path (GEMM kernels) for n <= workspace
+// batch rows of features X; Y out [members][n][3].
+int rapt_mlp_forward(rapt_mlp* m, int64_t n, const float* X, float* Y) {
+  if (!m || n <= 0 || !X || !Y) return m ? set_err(m->ctx, RAPT_E_ARG, "bad forward call")
+                                         : RAPT_E_ARG;
+  rapt_ctx* c = m->ctx;
+  HIP_OK(c, hipSetDevice(c->device));
+  const int bt = (int)((n + 63) / 64 * 64);
+  int rc = mlp_work(m, bt);
+  if (rc) return rc;
+  hipStream_t st = c->stream;
+  hipLaunchKernelGGL(k_zero_f, dim3((unsigned)((3 * bt + 255) / 256)), dim3(256), 0, st, m->X,
+                     (int64_t)3 * bt);
+  HIP_OK(c, hipMemcpyAsync(m->X, X, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st));
+  mlp_forward(m, bt, false);
+  HIP_OK(c, hipGetLastError());
+  for (int k = 0; k < m->members; ++k)
+    HIP_OK(c, hipMemcpyAsync(Y + (size_t)k * n * 3, m->Y + (size_t)k * bt * 3,
+                             sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st));
+  HIP_OK(c, hipStreamSynchronize(st));
+  return RAPT_OK;
+}
+
+// Ensemble inference with the fused kernel.  query [n][3] (host); out [n][3]
+// physical (T_b [s], |omega_d| [rad/s], mirror latitude [deg]); flag [n]:
+// 1 where the members' relative spread exceeds thresh on any output; raw
+// (optional) [members][n][3] normalised member outputs.  kernel_ms: time of
+// the inference kernel alone.
+int rapt_mlp_infer(rapt_mlp* m, int64_t n, const float* query, float thresh, float* out,
+                   int32_t* flag, float* raw, double* kernel_ms) {
+  if (!m || n <= 0 || !query || !out || !flag)
+    return m ? set_err(m->ctx, RAPT_E_ARG, "bad infer call") : RAPT_E_ARG;
+  rapt_ctx* c = m->ctx;
+  HIP_OK(c, hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  if (!m->packed_ok) {
+    for (int layer = 0; layer < 2; ++layer)
+      hipLaunchKernelGGL(k_pack, dim3(PACKED / 256, 1, m->members), dim3(256), 0, st, m->wb,
+                         m->packed + (int64_t)layer * PACKED, layer == 0 ? OFF_W2 : OFF_W3);
+    hipLaunchKernelGGL(k_pack1, dim3(8 * 64 * 8 / 256, 1, m->members), dim3(256), 0, st, m->wb,
+                       m->p1);
+    HIP_OK(c, hipGetLastError());
+    m->packed_ok = true;
+  }
+  float *dq = nullptr, *dout = nullptr, *draw = nullptr;
+  int32_t* dflag = nullptr;
+  int rc = RAPT_OK;
+  if (hipMalloc(&dq, sizeof(float) * 3 * n) != hipSuccess ||
+      hipMalloc(&dout, sizeof(float) * 3 * n) != hipSuccess ||
+      hipMalloc(&dflag, sizeof(int32_t) * n) != hipSuccess ||
+      (raw && hipMalloc(&draw, sizeof(float) * 3 * n * m->members) != hipSuccess)) {
+    rc = set_err(c, RAPT_E_OOM, "inference allocation failed");
+  } else if (hipMemcpyAsync(dq, query, sizeof(float) * 3 * n, hipMemcpyHostToDevice, st) !=
+             hipSuccess) {
+    rc = set_err(c, RAPT_E_HIP, "query copy failed");
+  } else {
+    InferArgs A;
+    A.n = n;
+    A.members = m->members;
+    A.query = dq;
+    A.wb = m->wb;
+    A.packed = m->packed;
+    A.p1 = m->p1;
+    A.norm = m->norm;
+    A.mass = m->mass;
+    A.eV = m->eV;
+    A.thresh = thresh;
+    A.out = dout;
+    A.raw = draw;
+    A.flag = dflag;
+    int dev_cus = 256;
+    (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, c->device);
+    int64_t groups = (n + 31) / 32;
+    int64_t blocks = (groups + 3) / 4;
+    const int64_t cap = (int64_t)dev_cus * 8;
+    if (blocks > cap) blocks = cap;
+    HIP_OK(c, hipEventRecord(c->ev0, st));
+    hipLaunchKernelGGL(k_infer, dim3((unsigned)blocks), dim3(256), 0, st, A);
+    HIP_OK(c, hipEventRecord(c->ev1, st));
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(out, dout, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(flag, dflag, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        (raw && hipMemcpyAsync(raw, draw, sizeof(float) * 3 * n * m->members,
+                               hipMemcpyDeviceToHost, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = set_err(c, RAPT_E_HIP, "inference kernel failed");
+    } else if (kernel_ms) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+      *kernel_ms = ms;
+    }
+  }
+  (void)hipFree(dq);
+  (void)hipFree(dout);
+  (void)hipFree(dflag);
+  (void)hipFree(draw);
+  return rc;
+}
+
+}  // extern "C"
