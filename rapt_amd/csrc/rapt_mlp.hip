@@ -235,28 +235,34 @@ __global__ void k_act_bwd(int Bt, const float* dH, const float* Z, bf16* dZ, int
   dZ[mem * sZ + idx] = (bf16)(dH[mem * sZ + idx] * dsilu(Z[mem * sZ + idx]));
 }
 
-// column sums over the batch: g[goff + n] = sum_s dZ[s][n] (bias gradients)
-__global__ void k_colsum(int Bt, const bf16* dZ, float* grad, int goff, int64_t sZ) {
-  const int mem = blockIdx.z;
-  const int n = threadIdx.x;
+// Batch reductions in two deterministic stages: per chunk of RCH samples
+// (grid x = chunk), then a sum over chunks in chunk order.
+constexpr int RCH = 64;
+constexpr int NPART = 8;  // partial slots per column
+
+// bias gradient partials: part[mem][c][n][slot] = sum over chunk of dZ[s][n]
+__global__ void k_colsum_part(int Bt, const bf16* dZ, float* part, int slot, int64_t sZ) {
+  const int mem = blockIdx.z, c = blockIdx.x, n = threadIdx.x;
+  const int chunks = Bt / RCH;
+  const bf16* p = dZ + mem * sZ + (int64_t)c * RCH * H;
   float acc = 0.0f;
-  const bf16* p = dZ + mem * sZ;
-  for (int s = 0; s < Bt; ++s) acc += (float)p[(int64_t)s * H + n];
-  grad[(int64_t)mem * NPAD + goff + n] = acc;
+  for (int s = 0; s < RCH; ++s) acc += (float)p[(int64_t)s * H + n];
+  part[(((int64_t)mem * chunks + c) * H + n) * NPART + slot] = acc;
 }
 
-// small reductions: dW4[k][n] = sum_s dY[s][k] H3[s][n], db4, dW1[n][i] =
-// sum_s dZ1[s][n] X[s][i] (dZ1 = dH1 silu'(Z1) formed here in fp32)
-__global__ void k_small_grads(int Bt, const float* dY, const float* H3, const float* dH1,
-                              const float* Z1, const float* X, float* grad, int64_t sZ) {
-  const int mem = blockIdx.z;
-  const int n = threadIdx.x;  // 256 threads
+// partials of dW4[k][n] = sum_s dY[s][k] H3[s][n] (slots 0..2), dW1[n][i] =
+// sum_s dZ1[s][n] X[s][i] (slots 3..5, dZ1 = dH1 silu'(Z1) in fp32), db1
+// (slot 6) and db4 (slot 7, columns 0..2)
+__global__ void k_small_part(int Bt, const float* dY, const float* H3, const float* dH1,
+                             const float* Z1, const float* X, float* part, int64_t sZ) {
+  const int mem = blockIdx.z, c = blockIdx.x, n = threadIdx.x;
+  const int chunks = Bt / RCH;
   const float* h3 = H3 + mem * sZ;
   const float* dh1 = dH1 + mem * sZ;
   const float* z1 = Z1 + mem * sZ;
   const float* dy = dY + (int64_t)mem * Bt * 3;
   float w4[3] = {0, 0, 0}, w1[3] = {0, 0, 0}, b1 = 0.0f, b4 = 0.0f;
-  for (int s = 0; s < Bt; ++s) {
+  for (int s = c * RCH; s < (c + 1) * RCH; ++s) {
     const float hv = h3[(int64_t)s * H + n];
     for (int k = 0; k < 3; ++k) w4[k] += dy[s * 3 + k] * hv;
     const float dz = dh1[(int64_t)s * H + n] * dsilu(z1[(int64_t)s * H + n]);
@@ -264,11 +270,40 @@ __global__ void k_small_grads(int Bt, const float* dY, const float* H3, const fl
     for (int i = 0; i < 3; ++i) w1[i] += dz * X[s * 3 + i];
     if (n < 3) b4 += dy[s * 3 + n];
   }
+  float* q = part + (((int64_t)mem * chunks + c) * H + n) * NPART;
+  q[0] = w4[0]; q[1] = w4[1]; q[2] = w4[2];
+  q[3] = w1[0]; q[4] = w1[1]; q[5] = w1[2];
+  q[6] = b1;
+  q[7] = b4;
+}
+
+// second stages: sum the partials over chunks and scatter the gradients
+// (k_small_part's slots; k_colsum_part's slot 0 = b3, slot 1 = b2)
+__global__ void k_sum_small(int chunks, const float* part, float* grad) {
+  const int mem = blockIdx.z, n = threadIdx.x;
+  float v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < chunks; ++c) {
+    const float* q = part + (((int64_t)mem * chunks + c) * H + n) * NPART;
+    for (int k = 0; k < NPART; ++k) v[k] += q[k];
+  }
   float* g = grad + (int64_t)mem * NPAD;
-  for (int k = 0; k < 3; ++k) g[OFF_W4 + k * H + n] = w4[k];
-  for (int i = 0; i < 3; ++i) g[OFF_W1 + n * 3 + i] = w1[i];
-  g[OFF_B1 + n] = b1;
-  if (n < 3) g[OFF_B4 + n] = b4;
+  for (int k = 0; k < 3; ++k) g[OFF_W4 + k * H + n] = v[k];
+  for (int i = 0; i < 3; ++i) g[OFF_W1 + n * 3 + i] = v[3 + i];
+  g[OFF_B1 + n] = v[6];
+  if (n < 3) g[OFF_B4 + n] = v[7];
+}
+
+__global__ void k_sum_bias(int chunks, const float* part, float* grad) {
+  const int mem = blockIdx.z, n = threadIdx.x;
+  float b3 = 0.0f, b2 = 0.0f;
+  for (int c = 0; c < chunks; ++c) {
+    const float* q = part + (((int64_t)mem * chunks + c) * H + n) * NPART;
+    b3 += q[0];
+    b2 += q[1];
+  }
+  float* g = grad + (int64_t)mem * NPAD;
+  g[OFF_B3 + n] = b3;
+  g[OFF_B2 + n] = b2;
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 masters; refreshes the BF16 copy
@@ -555,14 +590,16 @@ struct rapt_mlp {
   float *X = nullptr, *T = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr;
   float *H3 = nullptr, *dH = nullptr, *Y = nullptr, *dY = nullptr;
   bf16 *H1 = nullptr, *H2 = nullptr, *dZb = nullptr;
+  float *partS = nullptr, *partB = nullptr;
   double* loss = nullptr;
 };
 
 static void mlp_free_work(rapt_mlp* m) {
   void* p[] = {m->X, m->T, m->Z1, m->Z2, m->Z3, m->H3, m->dH, m->Y, m->dY,
-               m->H1, m->H2, m->dZb, m->loss};
+               m->H1, m->H2, m->dZb, m->partS, m->partB, m->loss};
   for (void* q : p) (void)hipFree(q);
   m->X = m->T = m->Z1 = m->Z2 = m->Z3 = m->H3 = m->dH = m->Y = m->dY = nullptr;
+  m->partS = m->partB = nullptr;
   m->H1 = m->H2 = m->dZb = nullptr;
   m->loss = nullptr;
   m->batch = 0;
@@ -585,6 +622,8 @@ static int mlp_work(rapt_mlp* m, int bt) {
             hipMalloc(&m->H1, sizeof(bf16) * act) == hipSuccess &&
             hipMalloc(&m->H2, sizeof(bf16) * act) == hipSuccess &&
             hipMalloc(&m->dZb, sizeof(bf16) * act) == hipSuccess &&
+            hipMalloc(&m->partS, sizeof(float) * (act / RCH) * NPART) == hipSuccess &&
+            hipMalloc(&m->partB, sizeof(float) * (act / RCH) * NPART) == hipSuccess &&
             hipMalloc(&m->loss, sizeof(double) * m->members) == hipSuccess;
   if (!ok) {
     mlp_free_work(m);
@@ -620,28 +659,30 @@ static void mlp_backward(rapt_mlp* m, int bt) {
   hipStream_t st = m->ctx->stream;
   const int mem = m->members;
   const int64_t sZ = (int64_t)bt * H;
+  const int chunks = bt / RCH;
   const dim3 ew((unsigned)((sZ + 255) / 256), 1, mem);
   const dim3 gw(H / 64, H / 64, mem);   // 256 x 256 weight gradients
   const dim3 ga(H / 64, bt / 64, mem);  // batch x 256 activation gradients
+  const dim3 gr(chunks, 1, mem);        // per-chunk partial reductions
   // layer 3
   hipLaunchKernelGGL(k_out_bwd, ew, dim3(256), 0, st, bt, m->dY, m->wb, m->Z3, m->dZb, sZ);
   hipLaunchKernelGGL((k_gemm<true, false>), gw, dim3(256), 0, st, H, H, bt, m->dZb, H, sZ,
                      m->H2, H, sZ, m->grad + OFF_W3, H, (int64_t)NPAD);
-  hipLaunchKernelGGL(k_colsum, dim3(1, 1, mem), dim3(256), 0, st, bt, m->dZb, m->grad, OFF_B3,
-                     sZ);
+  hipLaunchKernelGGL(k_colsum_part, gr, dim3(256), 0, st, bt, m->dZb, m->partB, 0, sZ);
   hipLaunchKernelGGL((k_gemm<false, false>), ga, dim3(256), 0, st, bt, H, H, m->dZb, H, sZ,
                      m->wb + OFF_W3, H, (int64_t)NPAD, m->dH, H, sZ);
   // layer 2
   hipLaunchKernelGGL(k_act_bwd, ew, dim3(256), 0, st, bt, m->dH, m->Z2, m->dZb, sZ);
   hipLaunchKernelGGL((k_gemm<true, false>), gw, dim3(256), 0, st, H, H, bt, m->dZb, H, sZ,
                      m->H1, H, sZ, m->grad + OFF_W2, H, (int64_t)NPAD);
-  hipLaunchKernelGGL(k_colsum, dim3(1, 1, mem), dim3(256), 0, st, bt, m->dZb, m->grad, OFF_B2,
-                     sZ);
+  hipLaunchKernelGGL(k_colsum_part, gr, dim3(256), 0, st, bt, m->dZb, m->partB, 1, sZ);
+  hipLaunchKernelGGL(k_sum_bias, dim3(1, 1, mem), dim3(256), 0, st, chunks, m->partB, m->grad);
   hipLaunchKernelGGL((k_gemm<false, false>), ga, dim3(256), 0, st, bt, H, H, m->dZb, H, sZ,
                      m->wb + OFF_W2, H, (int64_t)NPAD, m->dH, H, sZ);
   // layer 1 and the output layer
-  hipLaunchKernelGGL(k_small_grads, dim3(1, 1, mem), dim3(256), 0, st, bt, m->dY, m->H3, m->dH,
-                     m->Z1, m->X, m->grad, sZ);
+  hipLaunchKernelGGL(k_small_part, gr, dim3(256), 0, st, bt, m->dY, m->H3, m->dH, m->Z1, m->X,
+                     m->partS, sZ);
+  hipLaunchKernelGGL(k_sum_small, dim3(1, 1, mem), dim3(256), 0, st, chunks, m->partS, m->grad);
 }
 
 extern "C" {

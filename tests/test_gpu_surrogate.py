@@ -106,5 +106,16 @@ def test_surrogate_training_and_fallback():
         pv, pflag, info = s.predict(qte, thresh=1e-3)
         fl = (pflag == 1) & good
         assert np.allclose(pv[fl], truth[fl], rtol=1e-12)
+        # what the network gains over tracing: the same 65,536 queries by
+        # both methods (inference kernel time against the traced labels)
+        qg = _queries(1 << 16, 3)
+        s.infer(qg[:4096])
+        _, _, _, ms_inf = s.infer(qg)
+        t0 = time.perf_counter()
+        S.traced_labels(f, qg)
+        t_tr = time.perf_counter() - t0
+        print("65,536 queries: surrogate %.3f ms, traced %.1f ms, gain %.0fx" % (
+            ms_inf, t_tr * 1e3, t_tr * 1e3 / ms_inf))
+        assert t_tr * 1e3 / ms_inf > 50
     finally:
         s.close()
