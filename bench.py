@@ -445,7 +445,10 @@ def main():
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--lpp", type=int, default=4, choices=(1, 4),
-                    help="lanes per guiding centre (4: one lane quad per particle)")
+                    help="lanes per guiding centre (4: one lane quad per particle); "
+                         "with --no-tune")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the launch-shape autotuner (use --lpp/--lanes as given)")
     ap.add_argument("--cpu-sample", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", choices=("input", "dt"), default="dt",
@@ -487,6 +490,14 @@ def main():
     cap = BT._rows_needed(w["delta"], w["dt"])
     desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes,
                        lanes_per_particle=args.lpp, work_order=0 if args.order == "dt" else 1)
+    tuned = None
+    if not args.no_tune:
+        from rapt_amd import autotune as AT
+        shape, table = AT.tune(w["field"], S, flags, cap, desc, log=log)
+        AT.apply(desc, shape)
+        tuned = {"chosen": shape, "trials": [{"shape": t, "steps_per_s": r} for t, r in table],
+                 "trial_fraction_of_delta": 0.1}
+        log("autotune chose %s" % shape)
     b = N.Batch(ctx, n)
     b.upload(S, flags)
     b.set_rows(5, cap)
@@ -540,10 +551,12 @@ def main():
                         "L U[3,8], eq. pitch U[15,85] deg; TCB EOM, DOPRI5 rtol=atol=1e-8, "
                         "advance(60), dt=bounceperiod/10" % n,
             "particles": n, "delta_s": 60.0, "rtol": 1e-8, "atol": 1e-8,
-            "lanes": args.lanes or n * args.lpp, "lanes_per_particle": args.lpp,
+            "lanes": desc.lanes or n * max(desc.lanes_per_particle, 1),
+            "lanes_per_particle": desc.lanes_per_particle or 4, "block": desc.block or 256,
             "order": args.order,
         },
         "particles_done": ok,
+        "autotune": tuned,
         "accepted_steps_per_step": tot["naccpt"] / launches,
         "rhs_evals_per_step": tot["nfcn"] / launches,
         "rejected_steps_per_step": tot["nrejct"] / launches,

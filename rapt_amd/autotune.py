@@ -1,0 +1,84 @@
+"""Launch-shape autotuner for the batched tracers.
+
+The best launch shape depends on the batch: at 65,536 guiding centres the
+four-lane kernel wins (one lane per particle leaves one wave per SIMD and the
+slowest particle's chain sets the time), at 1,048,576 the one-lane kernel
+wins (enough particles to fill the chip, and it spends fewer issue slots per
+step).  The tuner times short trial launches of the real batch -- the same
+particles, the same work order, a fraction of the advance -- for each
+candidate (block size, lanes per particle, resident lanes, i.e. the work
+refill threshold) and keeps the one with the highest accepted-steps rate.
+Results are cached per (mode, stepper, field model, precision, equations,
+batch size rounded to a power of two).
+
+Trials reuse one uploaded batch and restore its state on the device between
+runs (rapt_batch_reset), so the batch is traced from the same start each
+time and the caller's run is unaffected.
+"""
+import math
+
+import numpy as np
+
+from . import _native as N
+
+_cache = {}
+
+
+def candidates(mode, stepper, precision, n):
+    """Candidate launch shapes: dicts of rapt_trace_desc fields."""
+    out = []
+    quad_ok = (mode == N.MODE_GC and stepper == N.STEP_DOPRI5 and precision == N.PREC_FP64)
+    for block in (256, 128):
+        if quad_ok:
+            out.append(dict(lanes_per_particle=4, block=block, lanes=0))
+        out.append(dict(lanes_per_particle=1, block=block, lanes=0))
+    # fewer resident lanes than particles: refill from the queue sooner
+    if n >= 8 * 65536:
+        out.append(dict(lanes_per_particle=1, block=256, lanes=4 * 65536))
+    return out
+
+
+def key(desc, model, n):
+    return (desc.mode, desc.stepper, model, desc.precision, desc.eom,
+            int(math.log2(max(n, 1))))
+
+
+def tune(field, S, flags, row_cap, desc, trial_fraction=0.1, device=0, force=False, log=None):
+    """Choose a launch shape for this batch.  Returns (best dict, table of
+    (shape, accepted steps/s)).  desc is not modified."""
+    n = S.shape[1]
+    fd = field.descriptor()
+    k = key(desc, fd["model"], n)
+    if not force and k in _cache:
+        return _cache[k], []
+    ctx = N.context(device)
+    ctx.set_field(fd)
+    width = N.ROW_WIDTH[desc.mode]
+    b = N.Batch(ctx, n)
+    table = []
+    try:
+        b.upload(S, flags)
+        b.set_rows(width, row_cap)
+        trial = N.TraceDesc.from_buffer_copy(desc)
+        trial.delta = desc.delta * trial_fraction
+        for cand in candidates(desc.mode, desc.stepper, desc.precision, n):
+            for f, v in cand.items():
+                setattr(trial, f, v)
+            b.reset()
+            c = b.run(trial)
+            rate = c.naccpt / max(c.kernel_ms, 1e-6) * 1e3
+            table.append((dict(cand), rate))
+            if log:
+                log("autotune %s: %.3g accepted steps/s (%.1f ms)" % (cand, rate, c.kernel_ms))
+    finally:
+        b.close()
+    best = max(table, key=lambda t: t[1])[0]
+    _cache[k] = best
+    return best, table
+
+
+def apply(desc, shape):
+    """Copy a tuned shape into a rapt_trace_desc."""
+    for f, v in shape.items():
+        setattr(desc, f, v)
+    return desc

@@ -396,6 +396,26 @@ def test_precision_rejects_unknown_value():
         b.close()
 
 
+def test_autotuner_picks_a_candidate_and_keeps_results():
+    """The autotuner returns one of its candidate shapes, and a run with the
+    tuned shape gives the same rows as the default launch."""
+    from rapt_amd import autotune as AT
+    f = F.DoubleDipole()
+    n = 256
+    last, mu, v = _gc_batch(f, n)
+    prm = P(solvertolerances=(1e-8, 1e-8))
+    S = batch.gc_state(last, K.m_el, -K.e, mu, v, 0.05)
+    cap = batch._rows_needed(0.5, np.full(n, 0.05))
+    desc = N.make_desc(N.MODE_GC, prm, 0.5)
+    shape, table = AT.tune(f, S, np.zeros(n, np.int32), cap, desc, force=True)
+    assert shape in [t for t, _ in table]
+    assert shape in AT.candidates(desc.mode, desc.stepper, desc.precision, n)
+    a = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 0.5, params=prm)
+    b = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 0.5, params=prm, **shape)
+    for x, y in zip(a.rows, b.rows):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_work_refill_is_exact():
     """Fewer resident lanes than particles (work refill) gives the same rows."""
     wl = W.dipole_protons_1mev(n=200)
