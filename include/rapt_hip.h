@@ -171,6 +171,23 @@ int rapt_bounce_labels(rapt_ctx *ctx, int64_t n, const double *query, double mas
                        double charge, double Re, double eV, double eyegradientstep,
                        double fieldlineresolution, double *out, int32_t *status);
 
+/* BounceCenter.advance (rapt/BounceCenter.py:206-251) for n bounce centres:
+ * dopri5 with scipy's call semantics on the drift gamma m v^2 / (q S_b B^2)
+ * grad I x B (the reference's deriv, :235-244), one integrate() per output
+ * row.  tpos [n][4]: trajectory[-1] in, final state out; Bm: mirror field;
+ * coef: gamma m v^2 / q; dt: output step (BCtimestep x bounce period);
+ * tcur: first row time; nrow: rows per centre (len(arange(tcur, tcur +
+ * delta, dt))); rows: [sum nrow][4] (t, x, y, z) with the reference's row
+ * times (the arange value, one step behind the state); status 1 ok, -2
+ * nsteps, -3 step too small, < -19 field-line failure.  Static dipole-type
+ * fields (models 1, 2, 8). */
+int rapt_bounce_center_advance(rapt_ctx *ctx, int64_t n, double *tpos, const double *Bm,
+                               const double *coef, const double *dt, const double *tcur,
+                               const int64_t *nrow, double *rows, int32_t *status,
+                               int64_t *naccpt, double rtol, double atol,
+                               double eyegradientstep, double fieldlineresolution,
+                               int32_t nsteps);
+
 /* ---- bounce-drift surrogate (no counterpart in the reference) ------------------
  * An ensemble of MLPs 3 -> 256 -> 256 -> 256 -> 3 (SiLU; BF16 weights, FP32
  * accumulation; hidden 256 x 256 layers on MFMA) that maps a query (L [Re],

@@ -1,9 +1,12 @@
 """Bounce-averaged drift tracer, API of rapt/BounceCenter.py.
 
-Construction and accessors follow rapt/BounceCenter.py:74-316.  Its drift
-right-hand side needs 5-7 field-line integrals per evaluation
-(rapt/BounceCenter.py:235-244); that device kernel is not built yet, so
-``advance`` raises NotImplementedError rather than running on the CPU.
+Construction and accessors follow rapt/BounceCenter.py:74-316.  ``advance``
+runs on the GPU (rapt_bounce_center_advance): dopri5 on the drift
+gamma m v^2 / (q S_b B^2) grad I x B, whose S_b and grad I are field-line
+integrals evaluated on the device for every stage
+(rapt/BounceCenter.py:206-251).  The output step is params['BCtimestep']
+times the bounce period at the current point (host restatement of
+flutils.bounceperiod, as the reference computes it once per call).
 """
 import pickle
 
@@ -34,8 +37,35 @@ class BounceCenter:
             assert self.mu > 0
 
     def advance(self, delta):
-        raise NotImplementedError("BounceCenter.advance: the GPU bounce-averaged drift kernel "
-                                  "is not built yet")
+        """Advance for delta seconds (rapt/BounceCenter.py:206-251).  Rows are
+        appended at the times np.arange(tcur, tcur + delta, dt), each holding
+        the state one output step later, as the reference records them."""
+        from . import _native as N
+        from . import flutils
+        from . import params
+        from .constants import c
+        if self.isequatorial:
+            raise NotImplementedError("the equatorial drift branch (rapt/BounceCenter.py:237) "
+                                      "is not built; the reference never sets isequatorial")
+        v = self.v
+        gamma = 1.0 / np.sqrt(1 - (v / c) ** 2)
+        Bm = self.mass * gamma ** 2 * v ** 2 / (2 * self.mu)
+        bp = flutils.bounceperiod(self.trajectory[-1, :4], self.field, Bm, v)
+        dt = params["BCtimestep"] * bp
+        nrow = len(np.arange(self.tcur, self.tcur + delta, dt))
+        rtol, atol = params["solvertolerances"]
+        rows, _, st, _ = N.context().bounce_center_advance(
+            self.field.descriptor(), self.trajectory[-1, :4], Bm,
+            gamma * self.mass * v * v / self.charge, dt, self.tcur, nrow, rtol, atol,
+            params["eyegradientstep"], params["fieldlineresolution"])
+        if st[0] != 1:
+            import warnings
+            warnings.warn("BounceCenter.advance stopped early (status %d)" % st[0])
+        r = rows[0]
+        if st[0] != 1:
+            r = r[:np.count_nonzero(np.any(r != 0, axis=1))]
+        self.trajectory = np.vstack((self.trajectory, r))
+        self.tcur = self.trajectory[-1, 0]
 
     def save(self, filename):
         with open(filename, "wb") as f:

@@ -87,6 +87,9 @@ def lib():
     L.rapt_halfbounce.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
     L.rapt_bounce_labels.argtypes = [vp, C.c_int64, dp, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, dp, i32p]
+    L.rapt_bounce_center_advance.argtypes = [vp, C.c_int64, dp, dp, dp, dp, dp, i64p, dp, i32p,
+                                             i64p, C.c_double, C.c_double, C.c_double,
+                                             C.c_double, C.c_int32]
     fp = C.POINTER(C.c_float)
     L.rapt_mlp_create.argtypes = [vp, C.c_int32, fp, C.c_double, C.c_double, C.POINTER(vp)]
     L.rapt_mlp_destroy.argtypes = [vp]
@@ -107,7 +110,8 @@ def lib():
     L.rapt_trace.argtypes = [vp, C.POINTER(TraceDesc), C.c_int64, dp, i32p, i64p, dp, i64p,
                              i32p, C.POINTER(Counters)]
     for nm in ("rapt_ctx_create", "rapt_ctx_destroy", "rapt_field_set", "rapt_field_eval",
-               "rapt_halfbounce", "rapt_bounce_labels", "rapt_mlp_create", "rapt_mlp_destroy",
+               "rapt_halfbounce", "rapt_bounce_labels", "rapt_bounce_center_advance",
+               "rapt_mlp_create", "rapt_mlp_destroy",
                "rapt_mlp_set_norm", "rapt_mlp_targets", "rapt_mlp_train", "rapt_mlp_get_params",
                "rapt_mlp_forward", "rapt_mlp_infer",
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
@@ -202,6 +206,26 @@ class Context:
                                             float(fieldlineresolution), _dp(out), _ip32(st)),
                    "rapt_bounce_labels")
         return out, st
+
+
+    def bounce_center_advance(self, desc, tpos, Bm, coef, dt, tcur, nrow, rtol, atol,
+                              eyegradientstep, fieldlineresolution, nsteps=500):
+        """Advance n bounce centres; returns (rows list, final tpos, status, naccpt)."""
+        self.set_field(desc)
+        tp = np.ascontiguousarray(np.atleast_2d(tpos), dtype=np.float64).copy()
+        n = tp.shape[0]
+        f = lambda a: np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.float64)
+        Bm, coef, dt, tcur = f(Bm), f(coef), f(dt), f(tcur)
+        nrow = np.ascontiguousarray(np.broadcast_to(nrow, (n,)), dtype=np.int64)
+        rows = np.zeros((max(int(nrow.sum()), 1), 4))
+        st = np.zeros(n, np.int32)
+        acc = np.zeros(n, np.int64)
+        self.check(lib().rapt_bounce_center_advance(
+            self.h, n, _dp(tp), _dp(Bm), _dp(coef), _dp(dt), _dp(tcur), _ip64(nrow), _dp(rows),
+            _ip32(st), _ip64(acc), float(rtol), float(atol), float(eyegradientstep),
+            float(fieldlineresolution), int(nsteps)), "rapt_bounce_center_advance")
+        off = np.r_[0, np.cumsum(nrow)]
+        return [rows[off[i]:off[i + 1]] for i in range(n)], tp, st, acc
 
 
 def context(device=0):

@@ -25,6 +25,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <vector>
+
 #include "../../include/rapt_hip.h"
 #include "rapt_fields.hpp"
 #include "rapt_internal.hpp"
@@ -452,6 +454,160 @@ __global__ __launch_bounds__(256) void k_bounce_labels(BounceArgs A, LabelArgs L
   }
 }
 
+// ---- BounceCenter.advance (rapt/BounceCenter.py:206-251) -----------------------
+// The drift right-hand side of the reference's deriv (:235-244, the
+// non-equatorial branch): gamma m v^2 / (q S_b |B|^2) grad I x B, with S_b and
+// grad I from the field-line integrals above.  A failed line integral stores
+// its status and makes the derivative NaN.
+template <int M>
+struct DriftRHS {
+  const BounceArgs& A;
+  int64_t lane;
+  double Bm, coef, dstep;
+  int32_t st;
+  int64_t nf;
+  __device__ void operator()(double t, const double* Y, double* out) {
+    ++nf;
+    const FieldC& F = A.F;
+    const V3 B = fieldB<M>(F, t, Y[0], Y[1], Y[2]);
+    const double bsq = dot3(B, B);
+    double sb, ic;
+    int32_t s = line_integrals<M>(A, lane, t, Y[0], Y[1], Y[2], Bm, &sb, &ic, nullptr);
+    // gradI directions (rapt/flutils.py:196-207)
+    const double mx = fmax(__builtin_fabs(Y[0]), __builtin_fabs(Y[1]));
+    V3 r{Y[0] / mx, Y[1] / mx, 0.0};
+    const double ib = 1.0 / __builtin_sqrt(bsq);
+    const V3 b{B.x * ib, B.y * ib, B.z * ib};
+    const double rb = dot3(r, b);
+    r = V3{r.x - rb * b.x, r.y - rb * b.y, r.z - rb * b.z};
+    const double rn = 1.0 / __builtin_sqrt(dot3(r, r));
+    const V3 v1{r.x * rn, r.y * rn, r.z * rn};
+    const V3 v2 = cross3(b, v1);
+    double dI[2] = {0.0, 0.0};
+    for (int k = 0; k < 2 && s == BST_OK; ++k) {
+      const V3 u = k == 0 ? v1 : v2;
+      double Ip = 0.0, Imn = 0.0;
+      s = line_integrals<M>(A, lane, t, Y[0] + dstep * u.x, Y[1] + dstep * u.y,
+                            Y[2] + dstep * u.z, Bm, nullptr, &Ip, nullptr);
+      if (s != BST_OK) break;
+      s = line_integrals<M>(A, lane, t, Y[0] - dstep * u.x, Y[1] - dstep * u.y,
+                            Y[2] - dstep * u.z, Bm, nullptr, &Imn, nullptr);
+      if (s != BST_OK) break;
+      if (Ip == 0.0) dI[k] = (ic - Imn) / dstep;
+      else if (Imn == 0.0) dI[k] = (Ip - ic) / dstep;
+      else dI[k] = (Ip - Imn) / (2.0 * dstep);
+    }
+    if (s != BST_OK) {
+      if (st == BST_OK) st = s;
+      out[0] = out[1] = out[2] = __builtin_nan("");
+      return;
+    }
+    const V3 gI{dI[0] * v1.x + dI[1] * v2.x, dI[0] * v1.y + dI[1] * v2.y,
+                dI[0] * v1.z + dI[1] * v2.z};
+    const double c = coef / (sb * bsq);
+    const V3 x = cross3(gI, B);
+    out[0] = c * x.x;
+    out[1] = c * x.y;
+    out[2] = c * x.z;
+  }
+};
+
+struct BCArgs {
+  int64_t n;
+  double* tpos;          // [n][4] (t, x, y, z): start in, end out
+  const double* Bm;      // [n] mirror field
+  const double* coef;    // [n] gamma m v^2 / q
+  const double* dt;      // [n] output step
+  const double* tcur;    // [n] first row time (np.arange start)
+  const int64_t* nrow;   // [n] rows to write
+  const int64_t* row_off;
+  double* rows;          // [.][4]
+  int32_t* status;
+  int64_t* naccpt;
+  double rtol, atol, dstep;
+  int32_t nmax, pad;
+  CtlParams c5;
+};
+
+// One lane per bounce centre: r = ode(deriv).set_integrator("dopri5", ...),
+// then for each t in arange(tcur, tcur + delta, dt): integrate(r.t + dt) and
+// append [t, r.y] (the row's time is the arange value, one step behind the
+// state, as the reference writes it).  scipy's call semantics as in k_trace.
+template <int M>
+__global__ __launch_bounds__(256) void k_bounce_center(BounceArgs A, BCArgs C) {
+  const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= A.lanes) return;
+  const CtlParams& P = C.c5;
+  for (int64_t i = lane; i < C.n; i += A.lanes) {
+    DriftRHS<M> f{A, lane, C.Bm[i], C.coef[i], C.dstep, BST_OK, 0};
+    double x = C.tpos[4 * i];
+    double y[3] = {C.tpos[4 * i + 1], C.tpos[4 * i + 2], C.tpos[4 * i + 3]};
+    double k1[3], y1[3], k7[3];
+    f(x, y, k1);
+    double h = 0.0;
+    int32_t code = 1;
+    int64_t acc = 0;
+    const double dt = C.dt[i];
+    for (int64_t r = 0; r < C.nrow[i] && code == 1; ++r) {
+      const double xend = x + dt;
+      const double hmax = xend - x;
+      if (h == 0.0) h = hinit<3>(f, x, y, k1, 5, hmax, C.atol, C.rtol);
+      double facold = 1.0e-4;
+      int nstep = 0;
+      bool last = false, reject = false;
+      for (;;) {
+        if (f.st != BST_OK) { code = f.st; break; }
+        if (nstep > C.nmax) { code = RAPT_ST_NSTEPS; break; }
+        if (0.1 * __builtin_fabs(h) <= __builtin_fabs(x) * UROUND) { code = RAPT_ST_HSMALL; break; }
+        if (x + 1.01 * h - xend > 0.0) {
+          h = xend - x;
+          last = true;
+        }
+        ++nstep;
+        const double err = dopri5_attempt<3>(f, x, h, y, k1, y1, k7, C.atol, C.rtol);
+        if (f.st != BST_OK) { code = f.st; break; }
+        const double fac11 = pow(err, P.expo1);
+        double fac = fac11 / pow(facold, P.beta);
+        fac = fmax(P.facc2, fmin(P.facc1, fac / P.safe));
+        double hnew = h / fac;
+        if (err <= 1.0) {
+          facold = fmax(err, 1.0e-4);
+          ++acc;
+          for (int k = 0; k < 3; ++k) {
+            k1[k] = k7[k];
+            y[k] = y1[k];
+          }
+          x = x + h;
+          if (last) {
+            h = hnew;
+            break;
+          }
+          if (__builtin_fabs(hnew) > hmax) hnew = hmax;
+          if (reject) hnew = fmin(__builtin_fabs(hnew), __builtin_fabs(h));
+          reject = false;
+          h = hnew;
+        } else {
+          hnew = h / fmin(P.facc1, fac11 / P.safe);
+          reject = true;
+          last = false;
+          h = hnew;
+        }
+      }
+      double* row = C.rows + (C.row_off[i] + r) * 4;
+      row[0] = C.tcur[i] + (double)r * dt;
+      row[1] = y[0];
+      row[2] = y[1];
+      row[3] = y[2];
+    }
+    C.tpos[4 * i] = x;
+    C.tpos[4 * i + 1] = y[0];
+    C.tpos[4 * i + 2] = y[1];
+    C.tpos[4 * i + 3] = y[2];
+    C.status[i] = code;
+    C.naccpt[i] = acc;
+  }
+}
+
 }  // namespace rapt
 
 using namespace rapt;
@@ -562,4 +718,135 @@ extern "C" int rapt_bounce_labels(rapt_ctx* c, int64_t n, const double* query, d
   if (n == 0) return RAPT_OK;
   const LabelArgs L{mass, charge, eyegradientstep, Re, eV};
   return run_bounce(c, n, query, 3, nullptr, fieldlineresolution, true, &L, out, 3, status);
+}
+
+template <int M>
+static void launch_bc(rapt_ctx* c, const BounceArgs& A, const BCArgs& C) {
+  const int B = 64;
+  hipLaunchKernelGGL(k_bounce_center<M>, dim3((unsigned)((A.lanes + B - 1) / B)), dim3(B), 0,
+                     c->stream, A, C);
+}
+
+extern "C" int rapt_bounce_center_advance(rapt_ctx* c, int64_t n, double* tpos, const double* Bm,
+                                          const double* coef, const double* dt,
+                                          const double* tcur, const int64_t* nrow, double* rows,
+                                          int32_t* status, int64_t* naccpt, double rtol,
+                                          double atol, double eyegradientstep,
+                                          double fieldlineresolution, int32_t nsteps) {
+  if (!c || n < 0 || (n && (!tpos || !Bm || !coef || !dt || !tcur || !nrow || !rows ||
+                            !status || !naccpt)))
+    return RAPT_E_ARG;
+  if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
+  if (!c->field.is_static)
+    return set_err(c, RAPT_E_ARG, "BounceCenter does not work with nonstatic fields");
+  if (!(fieldlineresolution > 0.0) || !(eyegradientstep > 0.0) || !(rtol > 0.0) ||
+      !(atol > 0.0))
+    return set_err(c, RAPT_E_ARG, "bad BounceCenter parameters");
+  if (n == 0) return RAPT_OK;
+  int64_t total = 0;
+  std::vector<int64_t> off((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (nrow[i] < 0 || !(dt[i] > 0.0)) return set_err(c, RAPT_E_ARG, "bad row count or step");
+    off[(size_t)i] = total;
+    total += nrow[i];
+  }
+  HIP_OK(c, hipSetDevice(c->device));
+  BounceArgs A;
+  memset(&A, 0, sizeof A);
+  A.F = c->field;
+  A.n = n;
+  A.res = fieldlineresolution;
+  A.cap = 2048;
+  A.lanes = n < 16384 ? n : 16384;
+  const size_t npts = (size_t)2 * A.cap * A.lanes;
+  double *d_tpos = nullptr, *d_in = nullptr, *d_rows = nullptr, *scratch = nullptr;
+  int64_t *d_nrow = nullptr, *d_off = nullptr, *d_acc = nullptr;
+  int32_t* d_st = nullptr;
+  int rc = RAPT_OK;
+  if (hipMalloc(&d_tpos, sizeof(double) * 4 * n) != hipSuccess ||
+      hipMalloc(&d_in, sizeof(double) * 4 * n) != hipSuccess ||
+      hipMalloc(&d_rows, sizeof(double) * 4 * (total > 0 ? total : 1)) != hipSuccess ||
+      hipMalloc(&d_nrow, sizeof(int64_t) * n) != hipSuccess ||
+      hipMalloc(&d_off, sizeof(int64_t) * n) != hipSuccess ||
+      hipMalloc(&d_acc, sizeof(int64_t) * n) != hipSuccess ||
+      hipMalloc(&d_st, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc(&scratch, sizeof(double) * 4 * npts) != hipSuccess) {
+    rc = set_err(c, RAPT_E_OOM, "BounceCenter allocation failed");
+  } else {
+    A.S = scratch;
+    A.Bv = scratch + npts;
+    A.Cc = scratch + 2 * npts;
+    A.Wk = scratch + 3 * npts;
+    BCArgs C;
+    memset(&C, 0, sizeof C);
+    C.n = n;
+    C.tpos = d_tpos;
+    C.Bm = d_in;
+    C.coef = d_in + n;
+    C.dt = d_in + 2 * n;
+    C.tcur = d_in + 3 * n;
+    C.nrow = d_nrow;
+    C.row_off = d_off;
+    C.rows = d_rows;
+    C.status = d_st;
+    C.naccpt = d_acc;
+    C.rtol = rtol;
+    C.atol = atol;
+    C.dstep = eyegradientstep;
+    C.nmax = nsteps > 0 ? nsteps : 500;
+    // scipy's dopri5 defaults: safety 0.9, fac1 0.2, fac2 10, beta 0 -> 0.04
+    C.c5.safe = 0.9;
+    C.c5.facc1 = 1.0 / 0.2;
+    C.c5.facc2 = 1.0 / 10.0;
+    C.c5.beta = 0.04;
+    C.c5.expo1 = 0.2 - 0.04 * 0.75;
+    bool ok = hipMemcpyAsync(d_tpos, tpos, sizeof(double) * 4 * n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess &&
+              hipMemcpyAsync(d_in, Bm, sizeof(double) * n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess &&
+              hipMemcpyAsync(d_in + n, coef, sizeof(double) * n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess &&
+              hipMemcpyAsync(d_in + 2 * n, dt, sizeof(double) * n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess &&
+              hipMemcpyAsync(d_in + 3 * n, tcur, sizeof(double) * n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess &&
+              hipMemcpyAsync(d_nrow, nrow, sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess &&
+              hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                             c->stream) == hipSuccess;
+    if (!ok) {
+      rc = set_err(c, RAPT_E_HIP, "BounceCenter input copy failed");
+    } else {
+      switch (c->field.model) {
+        case 1: launch_bc<1>(c, A, C); break;
+        case 2: launch_bc<2>(c, A, C); break;
+        case 8: launch_bc<8>(c, A, C); break;
+        default:
+          rc = set_err(c, RAPT_E_ARG, "BounceCenter needs a static dipole-type field (model %d)",
+                       c->field.model);
+      }
+      if (rc == RAPT_OK) {
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(tpos, d_tpos, sizeof(double) * 4 * n, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            (total > 0 && hipMemcpyAsync(rows, d_rows, sizeof(double) * 4 * total,
+                                         hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+            hipMemcpyAsync(status, d_st, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipMemcpyAsync(naccpt, d_acc, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+          rc = set_err(c, RAPT_E_HIP, "BounceCenter kernel failed");
+      }
+    }
+  }
+  (void)hipFree(d_tpos);
+  (void)hipFree(d_in);
+  (void)hipFree(d_rows);
+  (void)hipFree(d_nrow);
+  (void)hipFree(d_off);
+  (void)hipFree(d_acc);
+  (void)hipFree(d_st);
+  (void)hipFree(scratch);
+  return rc;
 }
