@@ -247,6 +247,22 @@ int rapt_batch_run(rapt_batch *b, const rapt_trace_desc *desc, rapt_counters *co
 int rapt_batch_download(rapt_batch *b, double *state, int32_t *flags, int32_t *status,
                         int64_t *nrows, double *rows, int64_t *naccpt);
 
+/* Output epochs: the batch runs in epochs of at most rows_per_epoch rows per
+ * particle, each epoch one launch of a captured hipGraph (counter reset,
+ * tracer kernel, copies of the epoch's rows, row counts and statuses to pinned
+ * host buffers).  Particles that stop on the full epoch buffer continue
+ * exactly in the next epoch; finished ones are skipped.  Device row memory is
+ * n x rows_per_epoch rows however long the run.  begin captures the graph for
+ * the descriptor (the batch's state, flags and work order as uploaded);
+ * next runs one epoch and returns pointers to the pinned buffers (valid until
+ * the next call): rows [n][rows_per_epoch][width], nrows [n] (rows written by
+ * this epoch), status [n], running = particles still to continue; end frees
+ * the epoch resources (also done by rapt_batch_destroy). */
+int rapt_batch_epochs_begin(rapt_batch *b, const rapt_trace_desc *desc, int32_t rows_per_epoch);
+int rapt_batch_epoch_next(rapt_batch *b, const double **rows, const int64_t **nrows,
+                          const int32_t **status, int32_t *running, rapt_counters *counters);
+int rapt_batch_epochs_end(rapt_batch *b);
+
 /* One-shot convenience: upload, run, download (host buffers). */
 int rapt_trace(rapt_ctx *ctx, const rapt_trace_desc *desc, int64_t n, double *state,
                int32_t *flags, const int64_t *row_cap, double *rows, int64_t *nrows,

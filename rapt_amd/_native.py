@@ -22,7 +22,7 @@ PREC_FP64, PREC_FP32_FIELD = 0, 1
 NSLOT = 16
 S_T, S_Y0, S_MASS, S_CHARGE, S_MU, S_V, S_DT, S_H, S_TEND, S_GM, S_TCUR = \
     0, 1, 7, 8, 9, 10, 11, 12, 13, 14, 15
-FLAG_STARTED, FLAG_GCMODE, FLAG_AINIT = 1, 2, 4
+FLAG_STARTED, FLAG_GCMODE, FLAG_AINIT, FLAG_FINAL = 1, 2, 4, 8
 ST_PENDING, ST_DONE, ST_SWITCH = 0, 1, 2
 ST_NSTEPS, ST_HSMALL, ST_NAN, ST_GCITER, ST_CAP, ST_HMIN = -2, -3, -4, -5, -6, -7
 STATUS_TEXT = {1: "ok", 2: "adiabaticity switch", -2: "larger nsteps is needed",
@@ -87,6 +87,10 @@ def lib():
     L.rapt_halfbounce.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
     L.rapt_bounce_labels.argtypes = [vp, C.c_int64, dp, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, dp, i32p]
+    L.rapt_batch_epochs_begin.argtypes = [vp, C.POINTER(TraceDesc), C.c_int32]
+    L.rapt_batch_epoch_next.argtypes = [vp, C.POINTER(dp), C.POINTER(i64p), C.POINTER(i32p),
+                                        C.POINTER(C.c_int32), C.POINTER(Counters)]
+    L.rapt_batch_epochs_end.argtypes = [vp]
     L.rapt_bounce_center_advance.argtypes = [vp, C.c_int64, dp, dp, dp, dp, dp, i64p, dp, i32p,
                                              i64p, C.c_double, C.c_double, C.c_double,
                                              C.c_double, C.c_int32]
@@ -111,6 +115,7 @@ def lib():
                              i32p, C.POINTER(Counters)]
     for nm in ("rapt_ctx_create", "rapt_ctx_destroy", "rapt_field_set", "rapt_field_eval",
                "rapt_halfbounce", "rapt_bounce_labels", "rapt_bounce_center_advance",
+               "rapt_batch_epochs_begin", "rapt_batch_epoch_next", "rapt_batch_epochs_end",
                "rapt_mlp_create", "rapt_mlp_destroy",
                "rapt_mlp_set_norm", "rapt_mlp_targets", "rapt_mlp_train", "rapt_mlp_get_params",
                "rapt_mlp_forward", "rapt_mlp_infer",
@@ -263,6 +268,32 @@ class Batch:
         assert state.shape == (NSLOT, self.n) and flags.shape == (self.n,)
         self.ctx.check(lib().rapt_batch_upload(self.h, _dp(state), _ip32(flags)),
                        "rapt_batch_upload")
+
+    def epochs(self, desc, rows_per_epoch, max_epochs=1 << 30):
+        """Run the batch in output epochs (rapt_batch_epochs_begin/epoch_next):
+        yields (rows [n][rows_per_epoch][width] copy, nrows [n], status [n],
+        running, Counters) per epoch until no particle is left running."""
+        width = ROW_WIDTH[desc.mode]
+        self.ctx.check(lib().rapt_batch_epochs_begin(self.h, C.byref(desc), int(rows_per_epoch)),
+                       "rapt_batch_epochs_begin")
+        try:
+            for _ in range(max_epochs):
+                r = C.POINTER(C.c_double)()
+                nr, st = C.POINTER(C.c_int64)(), C.POINTER(C.c_int32)()
+                left = C.c_int32(0)
+                cnt = Counters()
+                self.ctx.check(lib().rapt_batch_epoch_next(self.h, C.byref(r), C.byref(nr),
+                                                           C.byref(st), C.byref(left),
+                                                           C.byref(cnt)), "rapt_batch_epoch_next")
+                n, k = self.n, int(rows_per_epoch)
+                rows = np.ctypeslib.as_array(r, shape=(n * k * width,)).reshape(n, k, width).copy()
+                nrows = np.ctypeslib.as_array(nr, shape=(n,)).copy()
+                status = np.ctypeslib.as_array(st, shape=(n,)).copy()
+                yield rows, nrows, status, left.value, cnt
+                if left.value == 0:
+                    break
+        finally:
+            lib().rapt_batch_epochs_end(self.h)
 
     def reset(self):
         """Restore the uploaded state on the device (no host transfer)."""

@@ -148,7 +148,7 @@ void k_trace_gc4(KArgs A) {
       S[RAPT_S_TEND * n + pi] = tend;
       S[RAPT_S_GM * n + pi] = gm;
       S[RAPT_S_TCUR * n + pi] = tcur;
-      A.flags[pi] = FLAG_STARTED | FLAG_GCMODE;
+      A.flags[pi] = FLAG_STARTED | FLAG_GCMODE | (code != RAPT_ST_CAP ? FLAG_FINAL : 0);
       A.status[pi] = code;
       A.nrows[pi] = nrow;
       A.naccpt[pi] = p_acc;
@@ -166,6 +166,12 @@ void k_trace_gc4(KArgs A) {
       pi = qbll0(got);
       if (pi >= n) {
         exhausted = true;
+      } else if (A.flags[pi] & FLAG_FINAL) {
+        // finished in an earlier launch: no rows this time, status kept
+        if (q == 0) {
+          A.nrows[pi] = 0;
+          A.naccpt[pi] = 0;
+        }
       } else {
         active = true;
         x = S[0 * n + pi];
@@ -199,7 +205,7 @@ void k_trace_gc4(KArgs A) {
         }
       }
     }
-    if (__ballot(active) == 0) break;
+    if (__ballot(active || !exhausted) == 0) break;
 
     if (active && !k1ok) {
       k1 = rhs_gc_quad<M>(F, A.eom, charge, mass, mu, v, eq, q, x, y, c_fcn);

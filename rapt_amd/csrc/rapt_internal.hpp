@@ -18,6 +18,7 @@ constexpr int KIND_ADAPTIVE = 2;
 constexpr int FLAG_STARTED = 1;   // advance() in progress (state slots valid)
 constexpr int FLAG_GCMODE = 2;    // ADAPTIVE: current segment is a GuidingCenter
 constexpr int FLAG_AINIT = 4;     // ADAPTIVE: initial mode not decided yet
+constexpr int FLAG_FINAL = 8;     // advance() ended (any status but CAP): a relaunch skips it
 
 struct CtlParams {  // scipy set_integrator knobs of one stepper
   double safe, facc1, facc2, beta, expo1;

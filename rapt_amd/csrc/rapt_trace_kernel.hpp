@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
     S[RAPT_S_TEND * n + pi] = tend;
     S[RAPT_S_GM * n + pi] = gm;
     S[RAPT_S_TCUR * n + pi] = tcur;
-    int32_t fl = FLAG_STARTED | (gcmode ? FLAG_GCMODE : 0);
+    int32_t fl = FLAG_STARTED | (gcmode ? FLAG_GCMODE : 0) | (st != RAPT_ST_CAP ? FLAG_FINAL : 0);
     A.flags[pi] = fl;
     A.status[pi] = st;
     A.nrows[pi] = nrow;
@@ -168,6 +168,10 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
       if (pi < n && A.order) pi = A.order[pi];
       if (pi >= n) {
         exhausted = true;
+      } else if (A.flags[pi] & FLAG_FINAL) {
+        // finished in an earlier launch: no rows this time, status kept
+        A.nrows[pi] = 0;
+        A.naccpt[pi] = 0;
       } else {
         active = true;
         x = S[0 * n + pi];
@@ -223,7 +227,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         }
       }
     }
-    if (__ballot(active) == 0) break;
+    if (__ballot(active || !exhausted) == 0) break;
 
     // ---- f0 = f(x, y) at a call start without a carried FSAL stage ------------
     if (active && !k1ok) {

@@ -24,6 +24,20 @@
 #include "rapt_steppers.hpp"
 #include "rapt_trace_kernel.hpp"
 
+namespace rapt {
+// epoch graph nodes: counter reset (a fill kernel, not a memset node) and the
+// count of particles that stopped on a full epoch row buffer
+__global__ void k_epoch_prep(unsigned long long* work, int32_t* left) {
+  if (threadIdx.x < 8) work[threadIdx.x] = 0ULL;
+  if (threadIdx.x == 0) *left = 0;
+}
+__global__ void k_epoch_left(int64_t n, const int32_t* status, int32_t* left) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (status[i] == RAPT_ST_CAP) atomicAdd(left, 1);
+}
+}  // namespace rapt
+
 // ============================================================================
 // host side
 // ============================================================================
@@ -46,6 +60,20 @@ struct rapt_batch {
   int32_t width = 0;
   int64_t total_rows = 0;
   unsigned long long* d_work = nullptr;
+  // output epochs: a captured graph that runs one epoch of at most ep_rows
+  // rows per particle and copies its rows to pinned host memory
+  int32_t ep_rows = 0;
+  double* d_erows = nullptr;  // [n * ep_rows][width]
+  int64_t* d_eoff = nullptr;  // [n]: i * ep_rows
+  int64_t* d_ecap = nullptr;  // [n]: ep_rows
+  int32_t* d_left = nullptr;  // particles still running after an epoch
+  double* h_rows = nullptr;   // pinned copies of the epoch's outputs
+  int64_t* h_nrows = nullptr;
+  int32_t* h_status = nullptr;
+  int32_t* h_left = nullptr;
+  unsigned long long* h_work = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 };
 
 
@@ -238,6 +266,7 @@ int rapt_batch_destroy(rapt_batch* b) {
   (void)hipFree(b->d_rowcap);
   (void)hipFree(b->d_order);
   (void)hipFree(b->d_rows);
+  rapt_batch_epochs_end(b);
   (void)hipFree(b->d_work);
   delete b;
   return RAPT_OK;
@@ -322,7 +351,11 @@ int rapt_batch_set_rows(rapt_batch* b, int32_t width, const int64_t* row_cap) {
   return RAPT_OK;
 }
 
-int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) {
+}  // extern "C"
+
+// Validates a descriptor and builds the kernel arguments of a batch run.
+static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& block_out,
+                       int64_t& lanes_out, bool& quad_out) {
   if (!b || !d) return RAPT_E_ARG;
   rapt_ctx* c = b->ctx;
   if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
@@ -349,9 +382,9 @@ int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) 
   if (d->stepper == RAPT_STEP_RKF45 && !(d->rkf_hmax > 0.0 && d->rkf_hmin > 0.0))
     return set_err(c, RAPT_E_ARG, "RKF45 needs rkf_hmax, rkf_hmin > 0");
   const int block = d->block > 0 ? d->block : 256;
+  block_out = block;
   if (block % 64 != 0 || block > 256) return set_err(c, RAPT_E_ARG, "block must be 64..256, x64");
   HIP_OK(c, hipSetDevice(c->device));
-  KArgs A;
   memset(&A, 0, sizeof A);
   A.F = c->field;
   A.kind = d->mode;
@@ -409,8 +442,15 @@ int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) 
   // the four-lane kernel is FP64 only
   const bool quad = d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOPRI5 &&
                     d->lanes_per_particle != 1 && d->precision == RAPT_PREC_FP64;
-  HIP_OK(c, hipMemsetAsync(b->d_work, 0, sizeof(unsigned long long) * 8, c->stream));
-  HIP_OK(c, hipEventRecord(c->ev0, c->stream));
+  lanes_out = lanes;
+  quad_out = quad;
+  return RAPT_OK;
+}
+
+// The tracer kernel of a prepared run on the context's stream.
+static void launch_tracer(rapt_batch* b, const rapt_trace_desc* d, const KArgs& A, int block,
+                          int64_t lanes, bool quad) {
+  rapt_ctx* c = b->ctx;
   if (quad) {
     int64_t groups = d->lanes > 0 ? (d->lanes + 3) / 4 : b->n;
     if (groups > b->n) groups = b->n;
@@ -427,6 +467,22 @@ int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) 
       default: launch_model<KIND_ADAPTIVE>(c, A, block, lanes); break;
     }
   }
+}
+
+extern "C" {
+
+int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) {
+  if (!b || !d) return RAPT_E_ARG;
+  rapt_ctx* c = b->ctx;
+  KArgs A;
+  int block = 256;
+  int64_t lanes = 0;
+  bool quad = false;
+  const int rc = prepare_run(b, d, A, block, lanes, quad);
+  if (rc) return rc;
+  HIP_OK(c, hipMemsetAsync(b->d_work, 0, sizeof(unsigned long long) * 8, c->stream));
+  HIP_OK(c, hipEventRecord(c->ev0, c->stream));
+  launch_tracer(b, d, A, block, lanes, quad);
   HIP_OK(c, hipGetLastError());
   HIP_OK(c, hipEventRecord(c->ev1, c->stream));
   unsigned long long w[8];
@@ -488,6 +544,130 @@ int rapt_trace(rapt_ctx* c, const rapt_trace_desc* d, int64_t n, double* state, 
     rc = rapt_batch_download(b, state, flags, status, nrows, rows, nullptr);
   rapt_batch_destroy(b);
   return rc;
+}
+
+
+int rapt_batch_epochs_end(rapt_batch* b) {
+  if (!b) return RAPT_E_ARG;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  (void)hipFree(b->d_erows);
+  (void)hipFree(b->d_eoff);
+  (void)hipFree(b->d_ecap);
+  (void)hipFree(b->d_left);
+  (void)hipHostFree(b->h_rows);
+  (void)hipHostFree(b->h_nrows);
+  (void)hipHostFree(b->h_status);
+  (void)hipHostFree(b->h_left);
+  (void)hipHostFree(b->h_work);
+  b->d_erows = nullptr;
+  b->d_eoff = b->d_ecap = nullptr;
+  b->d_left = nullptr;
+  b->h_rows = nullptr;
+  b->h_nrows = nullptr;
+  b->h_status = nullptr;
+  b->h_left = nullptr;
+  b->h_work = nullptr;
+  b->ep_rows = 0;
+  return RAPT_OK;
+}
+
+int rapt_batch_epochs_begin(rapt_batch* b, const rapt_trace_desc* d, int32_t rows_per_epoch) {
+  if (!b || !d) return RAPT_E_ARG;
+  rapt_ctx* c = b->ctx;
+  if (rows_per_epoch < 1) return set_err(c, RAPT_E_ARG, "rows_per_epoch must be >= 1");
+  const int width = d->mode == RAPT_MODE_PARTICLE ? 7 : (d->mode == RAPT_MODE_GC ? 5 : 8);
+  rapt_batch_epochs_end(b);
+  HIP_OK(c, hipSetDevice(c->device));
+  const int64_t n = b->n;
+  const size_t nr = (size_t)n * rows_per_epoch;
+  bool ok = hipMalloc(&b->d_erows, sizeof(double) * nr * width) == hipSuccess &&
+            hipMalloc(&b->d_eoff, sizeof(int64_t) * n) == hipSuccess &&
+            hipMalloc(&b->d_ecap, sizeof(int64_t) * n) == hipSuccess &&
+            hipMalloc(&b->d_left, sizeof(int32_t)) == hipSuccess &&
+            hipHostMalloc(&b->h_rows, sizeof(double) * nr * width, 0) == hipSuccess &&
+            hipHostMalloc(&b->h_nrows, sizeof(int64_t) * n, 0) == hipSuccess &&
+            hipHostMalloc(&b->h_status, sizeof(int32_t) * n, 0) == hipSuccess &&
+            hipHostMalloc(&b->h_left, sizeof(int32_t), 0) == hipSuccess &&
+            hipHostMalloc(&b->h_work, sizeof(unsigned long long) * 8, 0) == hipSuccess;
+  if (!ok) {
+    rapt_batch_epochs_end(b);
+    return set_err(c, RAPT_E_OOM, "epoch buffers (%lld rows) do not fit", (long long)nr);
+  }
+  b->ep_rows = rows_per_epoch;
+  {
+    std::vector<int64_t> off((size_t)n), cap((size_t)n, rows_per_epoch);
+    for (int64_t i = 0; i < n; ++i) off[(size_t)i] = i * rows_per_epoch;
+    HIP_OK(c, hipMemcpy(b->d_eoff, off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
+    HIP_OK(c, hipMemcpy(b->d_ecap, cap.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice));
+  }
+  // the run's arguments, pointed at the epoch row buffer
+  const int32_t w0 = b->width;
+  b->width = width;
+  KArgs A;
+  int block = 256;
+  int64_t lanes = 0;
+  bool quad = false;
+  int rc = prepare_run(b, d, A, block, lanes, quad);
+  b->width = w0;
+  if (rc) {
+    rapt_batch_epochs_end(b);
+    return rc;
+  }
+  A.rows = b->d_erows;
+  A.row_off = b->d_eoff;
+  A.row_cap = b->d_ecap;
+  hipStream_t st = c->stream;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rapt_batch_epochs_end(b);
+    return set_err(c, RAPT_E_HIP, "graph capture failed to start");
+  }
+  hipLaunchKernelGGL(k_epoch_prep, dim3(1), dim3(64), 0, st, b->d_work, b->d_left);
+  launch_tracer(b, d, A, block, lanes, quad);
+  hipLaunchKernelGGL(k_epoch_left, dim3(64), dim3(256), 0, st, n, b->d_status, b->d_left);
+  (void)hipMemcpyAsync(b->h_rows, b->d_erows, sizeof(double) * nr * width, hipMemcpyDeviceToHost,
+                       st);
+  (void)hipMemcpyAsync(b->h_nrows, b->d_nrows, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st);
+  (void)hipMemcpyAsync(b->h_status, b->d_status, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+  (void)hipMemcpyAsync(b->h_left, b->d_left, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  (void)hipMemcpyAsync(b->h_work, b->d_work, sizeof(unsigned long long) * 8,
+                       hipMemcpyDeviceToHost, st);
+  if (hipStreamEndCapture(st, &b->graph) != hipSuccess ||
+      hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+    rapt_batch_epochs_end(b);
+    return set_err(c, RAPT_E_HIP, "graph capture failed");
+  }
+  return RAPT_OK;
+}
+
+int rapt_batch_epoch_next(rapt_batch* b, const double** rows, const int64_t** nrows,
+                          const int32_t** status, int32_t* running, rapt_counters* cnt) {
+  if (!b) return RAPT_E_ARG;
+  rapt_ctx* c = b->ctx;
+  if (!b->exec) return set_err(c, RAPT_E_STATE, "no epochs started (rapt_batch_epochs_begin)");
+  HIP_OK(c, hipSetDevice(c->device));
+  HIP_OK(c, hipEventRecord(c->ev0, c->stream));
+  HIP_OK(c, hipGraphLaunch(b->exec, c->stream));
+  HIP_OK(c, hipEventRecord(c->ev1, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  if (rows) *rows = b->h_rows;
+  if (nrows) *nrows = b->h_nrows;
+  if (status) *status = b->h_status;
+  if (running) *running = *b->h_left;
+  if (cnt) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    cnt->naccpt = (int64_t)b->h_work[1];
+    cnt->nrejct = (int64_t)b->h_work[2];
+    cnt->nstep = (int64_t)b->h_work[3];
+    cnt->nfcn = (int64_t)b->h_work[4];
+    cnt->nswitch = (int64_t)b->h_work[5];
+    cnt->nrows = 0;
+    cnt->kernel_ms = ms;
+  }
+  return RAPT_OK;
 }
 
 }  // extern "C"
