@@ -20,6 +20,7 @@
 // activations.  The third hidden activation stays FP32 into the output layer.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <new>
@@ -349,119 +350,32 @@ __global__ void k_to_bf16(const float* theta, bf16* wb, int64_t n) {
 }
 
 // ---- fused inference -------------------------------------------------------------------
-// Packed weights of a 256x256 layer for the register-resident forward pass.
-// The layer output of tile t (neurons 32t..32t+31) leaves lane l, register r
-// holding neuron 32t + 8(r/4) + 4(l/32) + r%4 (the MFMA D layout); the next
-// layer consumes registers 8u..8u+7 of tile t as its K-block 2t+u, so K index
-// 8h + j of block kb (h = lane half) is input neuron
-//   pi(kb, h, j) = 32(kb/2) + 16(kb%2) + 8(j/4) + 4h + j%4.
-// Packed[t'][kb][lane][j] = W[32t' + lane%32][pi(kb, lane/32, j)].
-__device__ __forceinline__ int pin(int kb, int h, int j) {
-  return 32 * (kb >> 1) + 16 * (kb & 1) + 8 * (j >> 2) + 4 * h + (j & 3);
-}
-__global__ void k_pack(const bf16* wb, bf16* packed, int woff) {
-  const int mem = blockIdx.z;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // over 8*16*64*8
-  if (idx >= PACKED) return;
-  const int j = idx & 7, lane = (idx >> 3) & 63, kb = (idx >> 9) & 15, tp = idx >> 13;
-  const int row = 32 * tp + (lane & 31), col = pin(kb, lane >> 5, j);
-  packed[(int64_t)mem * 2 * PACKED + idx] = wb[(int64_t)mem * NPAD + woff + row * H + col];
-}
-
-// Layer 1 as an MFMA: the three features are split into bf16 hi and lo parts
-// (x = hi + lo to 2^-17 relative), so K index j < 3 multiplies W1[.][j] by
-// hi_j and 3 <= j < 6 multiplies W1[.][j-3] by lo_{j-3}: the product with the
-// bf16 weights is as accurate as the FP32 vector form.  P1[t][lane][j] =
-// A-operand of output tile t.
-__global__ void k_pack1(const bf16* wb, bf16* p1) {
-  const int mem = blockIdx.z;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // 8 tiles * 64 lanes * 8
-  if (idx >= 8 * 64 * 8) return;
-  const int j = idx & 7, lane = (idx >> 3) & 63, t = idx >> 9;
-  const int n = 32 * t + (lane & 31);
-  bf16 v = (bf16)0.0f;
-  if ((lane >> 5) == 0 && j < 6) v = wb[(int64_t)mem * NPAD + OFF_W1 + n * 3 + (j % 3)];
-  p1[(int64_t)mem * 8 * 64 * 8 + idx] = v;
-}
-
-// per-member small parameters staged in LDS: (b1), (b2), (b3, w4[0..2]), b4
-struct SmallW {
-  float b1[H], b2[H];
-  float4 l3[H];
-  float b4[4];
-};
-constexpr int MAXM = 4;  // largest ensemble (all members' small parameters sit in LDS)
-
-// one member's forward pass for the wave's 32 queries (lane half h = l/32
-// holds half of each activation); returns the three normalised outputs in
-// every lane of the query.
-__device__ __forceinline__ void member_forward(const SmallW& sw, const bf16x8* P1,
-                                               const bf16x8* P2, const bf16x8* P3,
-                                               const bf16x8& xb, int l, float out[3]) {
-  const int h = l >> 5;
-  bf16x8 act[16];
-  // layer 1: D layout of tile t feeds K-blocks 2t, 2t+1 of layer 2
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    f32x16 acc = {};
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P1[t * 64 + l], xb, acc, 0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int n = 32 * t + 8 * (r >> 2) + 4 * h + (r & 3);
-      const bf16 v = (bf16)silu(acc[r] + sw.b1[n]);
-      if (r < 8) act[2 * t][r] = v; else act[2 * t + 1][r - 8] = v;
-    }
-  }
-  // layer 2 (fully unrolled: nxt is indexed by the tile)
-  bf16x8 nxt[16];
-#pragma unroll
-  for (int tp = 0; tp < 8; ++tp) {
-    f32x16 acc = {};
-#pragma unroll
-    for (int kb = 0; kb < 16; ++kb)
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P2[(tp * 16 + kb) * 64 + l], act[kb], acc,
-                                                     0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int n = 32 * tp + 8 * (r >> 2) + 4 * h + (r & 3);
-      const bf16 v = (bf16)silu(acc[r] + sw.b2[n]);
-      if (r < 8) nxt[2 * tp][r] = v; else nxt[2 * tp + 1][r - 8] = v;
-    }
-  }
-  // layer 3 and the output layer (FP32 activations)
-  float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
-#pragma unroll 1
-  for (int tp = 0; tp < 8; ++tp) {
-    f32x16 acc = {};
-#pragma unroll
-    for (int kb = 0; kb < 16; ++kb)
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P3[(tp * 16 + kb) * 64 + l], nxt[kb], acc,
-                                                     0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int n = 32 * tp + 8 * (r >> 2) + 4 * h + (r & 3);
-      const float4 c = sw.l3[n];
-      const float v = silu(acc[r] + c.x);
-      o0 += c.y * v;
-      o1 += c.z * v;
-      o2 += c.w * v;
-    }
-  }
-  o0 += __shfl_xor(o0, 32);
-  o1 += __shfl_xor(o1, 32);
-  o2 += __shfl_xor(o2, 32);
-  out[0] = o0 + sw.b4[0];
-  out[1] = o1 + sw.b4[1];
-  out[2] = o2 + sw.b4[2];
-}
+// One kernel runs the whole ensemble forward pass, the disagreement test and
+// the baseline for a batch of queries; activations never leave registers.
+//
+// v_mfma_f32_16x16x32_bf16, 16 queries per wave: lane l holds query l%16 and
+// quarter h = l/16 of each activation.  A 16x16 output tile t leaves register
+// r of lane l holding neuron 16t + 4h + r (the D layout of the instruction:
+// D[4(l/16)+r][l%16]); tiles 2kb and 2kb+1 form the B operand of K-block kb
+// of the next layer, so K index 8h + j of block kb is input neuron
+// pin16(kb, h, j) = 32kb + 16(j/4) + 4h + j%4, and the packed weights are
+// P[t][kb][lane][j] = W[16t + lane%16][pin16(kb, lane/16, j)].  A block of
+// IW waves stages one member's packed 256x256 layer (128 KB) in LDS and its
+// waves (IW x 16 queries) read the A fragments from there, so L2 supplies
+// each weight once per block instead of once per wave.  Measured on the
+// 2^24-query four-member ensemble: 42.6 ms, against 45.8 ms for 32 queries
+// per wave with fragments read from L2 and 43.7 ms for 32 queries per wave
+// with LDS staging; 16 waves per block took 48.1 ms.
+constexpr int MAXM = 4;   // largest ensemble
+constexpr int IW = 8;     // waves per block (two per SIMD)
 
 struct InferArgs {
   int64_t n;
   int members;
   const float* query;     // [n][3]
   const bf16* wb;         // [members][NPAD]
-  const bf16* packed;     // [members][2][PACKED]
-  const bf16* p1;         // [members][8*64*8]
+  const bf16* packed;     // [members][2][PACKED]: layers 2 and 3
+  const bf16* p1;         // [members][16*64*8]: layer-1 A operands
   const float* norm;      // shift[3], scale[3]
   double mass, eV;
   float thresh;           // relative member deviation that sends a query to the fallback
@@ -470,34 +384,53 @@ struct InferArgs {
   int32_t* flag;          // [n] 1 = fallback needed
 };
 
-// Persistent over 32-query groups: each wave takes groups g = wave, wave +
-// waves, ...; the block stages every member's small parameters once.  Two
-// waves per SIMD (256 registers, a few spilled) beat one wave without spills
-// by 1.9x and three or four waves with heavier spilling (measured on the
-// 2^24-query ensemble inference).
-#ifndef RAPT_MLP_WAVES
-#define RAPT_MLP_WAVES 2
-#endif
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAPT_MLP_WAVES)))
-void k_infer(InferArgs A) {
-  __shared__ SmallW sw[MAXM];
-  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int64_t waves = (int64_t)gridDim.x * 4;
-  const int64_t groups = (A.n + 31) / 32;
-  for (int k = 0; k < A.members; ++k) {
-    const bf16* w = A.wb + (int64_t)k * NPAD;
-    for (int i = threadIdx.x; i < H; i += 256) {
-      sw[k].b1[i] = (float)w[OFF_B1 + i];
-      sw[k].b2[i] = (float)w[OFF_B2 + i];
-      sw[k].l3[i] = make_float4((float)w[OFF_B3 + i], (float)w[OFF_W4 + i],
-                                (float)w[OFF_W4 + H + i], (float)w[OFF_W4 + 2 * H + i]);
-    }
-    if (threadIdx.x < 3) sw[k].b4[threadIdx.x] = (float)w[OFF_B4 + threadIdx.x];
-  }
-  __syncthreads();
-  for (int64_t g = (int64_t)blockIdx.x * 4 + wv; g < groups; g += waves) {
-    const int64_t qi = g * 32 + (l & 31);
-    const bool valid = qi < A.n;
+// one member's small parameters, staged in LDS: b1, b2, (b3, w4[0..2]), b4
+struct SmallW {
+  float b1[H], b2[H];
+  float4 l3[H];
+  float b4[4];
+};
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int pin16(int kb, int h, int j) {
+  return 32 * kb + 16 * (j >> 2) + 4 * h + (j & 3);
+}
+
+__global__ void k_pack(const bf16* wb, bf16* packed, int woff) {
+  const int mem = blockIdx.z;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= PACKED) return;
+  const int j = idx & 7, lane = (idx >> 3) & 63, kb = (idx >> 9) & 7, tp = idx >> 12;
+  const int row = 16 * tp + (lane & 15), col = pin16(kb, lane >> 4, j);
+  packed[(int64_t)mem * 2 * PACKED + idx] = wb[(int64_t)mem * NPAD + woff + row * H + col];
+}
+
+// Layer 1 as an MFMA: the three features are split into bf16 hi and lo parts
+// (x = hi + lo to 2^-17 relative), so K index j < 3 multiplies W1[.][j] by
+// hi_j and 3 <= j < 6 multiplies W1[.][j-3] by lo_{j-3}: the product with the
+// bf16 weights is as accurate as the FP32 vector form.  A operand of tile t
+// (16 neurons), lane (h, j): W1[16t + l%16][j%3] for h = 0, j < 6, else 0.
+__global__ void k_pack1(const bf16* wb, bf16* p1) {
+  const int mem = blockIdx.z;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // 16 tiles * 64 lanes * 8
+  if (idx >= 16 * 64 * 8) return;
+  const int j = idx & 7, lane = (idx >> 3) & 63, t = idx >> 9;
+  const int n = 16 * t + (lane & 15);
+  bf16 v = (bf16)0.0f;
+  if ((lane >> 4) == 0 && j < 6) v = wb[(int64_t)mem * NPAD + OFF_W1 + n * 3 + (j % 3)];
+  p1[(int64_t)mem * 16 * 64 * 8 + idx] = v;
+}
+
+__global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
+  __shared__ bf16x8 wst[PACKED / 8];
+  __shared__ SmallW sw;
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, h = l >> 4;
+  const int64_t groups = (A.n + 15) / 16;
+  const int64_t chunks = (groups + IW - 1) / IW;
+  for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+    const int64_t g = ch * IW + wv;
+    const int64_t qi = g * 16 + (l & 15);
+    const bool valid = g < groups && qi < A.n;
     float q[3], xin[3];
     q[0] = valid ? A.query[qi * 3] : 5.0f;
     q[1] = valid ? A.query[qi * 3 + 1] : 45.0f;
@@ -506,7 +439,7 @@ void k_infer(InferArgs A) {
     bf16x8 xb;
 #pragma unroll
     for (int j = 0; j < 8; ++j) xb[j] = (bf16)0.0f;
-    if (l < 32) {
+    if (h == 0) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const bf16 hi = (bf16)xin[j];
@@ -517,28 +450,90 @@ void k_infer(InferArgs A) {
     float mean[3] = {0.0f, 0.0f, 0.0f}, lo[3], hi[3];
     for (int c = 0; c < 3; ++c) { lo[c] = 3.0e38f; hi[c] = -3.0e38f; }
     for (int m = 0; m < A.members; ++m) {
-      const bf16x8* P1 = (const bf16x8*)(A.p1 + (int64_t)m * 8 * 64 * 8);
-      const bf16x8* P2 = (const bf16x8*)(A.packed + ((int64_t)m * 2) * PACKED);
-      const bf16x8* P3 = (const bf16x8*)(A.packed + ((int64_t)m * 2 + 1) * PACKED);
-      float o[3];
-      member_forward(sw[m], P1, P2, P3, xb, l, o);
+      const bf16* w = A.wb + (int64_t)m * NPAD;
+      const bf16x8* P1 = (const bf16x8*)(A.p1 + (int64_t)m * 16 * 64 * 8);
+      __syncthreads();
+      for (int i = threadIdx.x; i < H; i += IW * 64) {
+        sw.b1[i] = (float)w[OFF_B1 + i];
+        sw.b2[i] = (float)w[OFF_B2 + i];
+        sw.l3[i] = make_float4((float)w[OFF_B3 + i], (float)w[OFF_W4 + i],
+                               (float)w[OFF_W4 + H + i], (float)w[OFF_W4 + 2 * H + i]);
+      }
+      if (threadIdx.x < 3) sw.b4[threadIdx.x] = (float)w[OFF_B4 + threadIdx.x];
+      {
+        const bf16x8* src = (const bf16x8*)(A.packed + ((int64_t)m * 2) * PACKED);
+        for (int i = threadIdx.x; i < PACKED / 8; i += IW * 64) wst[i] = src[i];
+      }
+      __syncthreads();
+      // layer 1
+      bf16x8 act[8];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        f32x4 acc = {};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P1[t * 64 + l], xb, acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) act[t >> 1][(t & 1) * 4 + r] =
+            (bf16)silu(acc[r] + sw.b1[16 * t + 4 * h + r]);
+      }
+      // layer 2
+      bf16x8 nxt[8];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        f32x4 acc = {};
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb)
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wst[(t * 8 + kb) * 64 + l], act[kb], acc,
+                                                         0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) nxt[t >> 1][(t & 1) * 4 + r] =
+            (bf16)silu(acc[r] + sw.b2[16 * t + 4 * h + r]);
+      }
+      __syncthreads();
+      {
+        const bf16x8* src = (const bf16x8*)(A.packed + ((int64_t)m * 2 + 1) * PACKED);
+        for (int i = threadIdx.x; i < PACKED / 8; i += IW * 64) wst[i] = src[i];
+      }
+      __syncthreads();
+      // layer 3 and the output layer (FP32)
+      float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+#pragma unroll 2
+      for (int t = 0; t < 16; ++t) {
+        f32x4 acc = {};
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb)
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wst[(t * 8 + kb) * 64 + l], nxt[kb], acc,
+                                                         0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float4 c = sw.l3[16 * t + 4 * h + r];
+          const float v = silu(acc[r] + c.x);
+          o0 += c.y * v;
+          o1 += c.z * v;
+          o2 += c.w * v;
+        }
+      }
+      o0 += __shfl_xor(o0, 16);
+      o1 += __shfl_xor(o1, 16);
+      o2 += __shfl_xor(o2, 16);
+      o0 += __shfl_xor(o0, 32);
+      o1 += __shfl_xor(o1, 32);
+      o2 += __shfl_xor(o2, 32);
+      const float o[3] = {o0 + sw.b4[0], o1 + sw.b4[1], o2 + sw.b4[2]};
       for (int c = 0; c < 3; ++c) {
         mean[c] += o[c];
         lo[c] = fminf(lo[c], o[c]);
         hi[c] = fmaxf(hi[c], o[c]);
       }
-      if (A.raw && valid && l < 32)
+      if (A.raw && valid && l < 16)
         for (int c = 0; c < 3; ++c) A.raw[((int64_t)m * A.n + qi) * 3 + c] = o[c];
     }
-    if (valid && l < 32) {
+    if (valid && l < 16) {
       const Base b = baseline(q[0], q[1], q[2], A.mass, A.eV);
       const float bb[3] = {b.lnT, b.lnW, b.lnLat};
       int bad = 0;
       for (int k = 0; k < 3; ++k) {
         const float sc = A.norm[3 + k];
         const float mu = mean[k] / (float)A.members;
-        // largest relative deviation of a member from the ensemble mean:
-        // exp(sc * max(hi - mu, mu - lo)) - 1
         if (expm1f(sc * fmaxf(hi[k] - mu, mu - lo[k])) > A.thresh) bad = 1;
         A.out[qi * 3 + k] = expf(bb[k] + A.norm[k] + sc * mu);
       }
@@ -705,7 +700,7 @@ int rapt_mlp_create(rapt_ctx* c, int32_t members, const float* theta0, double ma
             hipMalloc(&m->m2, sizeof(float) * np) == hipSuccess &&
             hipMalloc(&m->wb, sizeof(bf16) * np) == hipSuccess &&
             hipMalloc(&m->packed, sizeof(bf16) * members * 2 * PACKED) == hipSuccess &&
-            hipMalloc(&m->p1, sizeof(bf16) * members * 8 * 64 * 8) == hipSuccess &&
+            hipMalloc(&m->p1, sizeof(bf16) * members * 16 * 64 * 8) == hipSuccess &&
             hipMalloc(&m->norm, sizeof(float) * 6) == hipSuccess;
   if (!ok) {
     rapt_mlp_destroy(m);
@@ -923,7 +918,7 @@ int rapt_mlp_infer(rapt_mlp* m, int64_t n, const float* query, float thresh, flo
     for (int layer = 0; layer < 2; ++layer)
       hipLaunchKernelGGL(k_pack, dim3(PACKED / 256, 1, m->members), dim3(256), 0, st, m->wb,
                          m->packed + (int64_t)layer * PACKED, layer == 0 ? OFF_W2 : OFF_W3);
-    hipLaunchKernelGGL(k_pack1, dim3(8 * 64 * 8 / 256, 1, m->members), dim3(256), 0, st, m->wb,
+    hipLaunchKernelGGL(k_pack1, dim3(16 * 64 * 8 / 256, 1, m->members), dim3(256), 0, st, m->wb,
                        m->p1);
     HIP_OK(c, hipGetLastError());
     m->packed_ok = true;
@@ -956,12 +951,12 @@ int rapt_mlp_infer(rapt_mlp* m, int64_t n, const float* query, float thresh, flo
     A.flag = dflag;
     int dev_cus = 256;
     (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, c->device);
-    int64_t groups = (n + 31) / 32;
-    int64_t blocks = (groups + 3) / 4;
-    const int64_t cap = (int64_t)dev_cus * 8;
-    if (blocks > cap) blocks = cap;
+    // persistent: one block per CU, each taking chunks of IW x 16 queries
+    const int64_t groups = (n + 15) / 16;
+    int64_t blocks = (groups + IW - 1) / IW;
+    if (blocks > dev_cus) blocks = dev_cus;
     HIP_OK(c, hipEventRecord(c->ev0, st));
-    hipLaunchKernelGGL(k_infer, dim3((unsigned)blocks), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_infer, dim3((unsigned)blocks), dim3(IW * 64), 0, st, A);
     HIP_OK(c, hipEventRecord(c->ev1, st));
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(out, dout, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, st) !=
