@@ -444,16 +444,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--lpp", type=int, default=4, choices=(1, 4),
-                    help="lanes per guiding centre (4: one lane quad per particle); "
-                         "with --no-tune")
+    ap.add_argument("--lpp", type=int, default=4, choices=(1, 2, 4),
+                    help="lanes per guiding centre (4: a lane quad, 2: a lane pair per "
+                         "particle); with --no-tune")
+    ap.add_argument("--block", type=int, default=0,
+                    help="threads per block (0: library default); with --no-tune")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (use --lpp/--lanes as given)")
     ap.add_argument("--cpu-sample", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--order", choices=("input", "dt"), default="dt",
-                    help="library work order: shortest output step (bounce period) first "
-                         "(default), or input order")
+    ap.add_argument("--order", choices=("cost", "dt", "input"), default="cost",
+                    help="library work order: descending cost measured by a trial launch "
+                         "over 10%% of the advance (default), shortest output step first, "
+                         "or input order")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -488,18 +491,27 @@ def main():
     S = BT.gc_state(w["last5"], w["m"], w["q"], w["mu"], w["v"], w["dt"])
     flags = np.zeros(n, np.int32)
     cap = BT._rows_needed(w["delta"], w["dt"])
-    desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes,
-                       lanes_per_particle=args.lpp, work_order=0 if args.order == "dt" else 1)
+    desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes, block=args.block,
+                       lanes_per_particle=args.lpp, work_order=1 if args.order == "input" else 0)
+    cost = None
+    if args.order == "cost":
+        from rapt_amd import autotune as AT
+        cost = AT.trial_cost(w["field"], S, flags, cap, desc)
+        log("work order: descending accepted steps of a 10%%-of-delta trial launch "
+            "(mean %.0f per particle)" % cost.mean())
     tuned = None
     if not args.no_tune:
         from rapt_amd import autotune as AT
-        shape, table = AT.tune(w["field"], S, flags, cap, desc, log=log)
+        shape, table = AT.tune(w["field"], S, flags, cap, desc, trial_fraction=0.25, log=log,
+                               cost=cost)
         AT.apply(desc, shape)
         tuned = {"chosen": shape, "trials": [{"shape": t, "steps_per_s": r} for t, r in table],
-                 "trial_fraction_of_delta": 0.1}
+                 "trial_fraction_of_delta": 0.25, "trial_work_order": args.order}
         log("autotune chose %s" % shape)
     b = N.Batch(ctx, n)
     b.upload(S, flags)
+    if cost is not None:
+        b.set_cost(cost)
     b.set_rows(5, cap)
     log("rows allocated: %d (%.2f GB)" % (cap.sum(), cap.sum() * 40 / 1e9))
 

@@ -96,8 +96,10 @@ typedef struct {
     double delta;                /* advance(delta) */
     int64_t lanes;               /* resident lanes of the persistent grid (0 = auto);
                                     fewer lanes than particles turns on work refill */
-    int32_t lanes_per_particle;  /* GC with DOPRI5: 4 (default, 0) spreads one guiding
-                                    centre over a lane quad; 1 = one lane per particle */
+    int32_t lanes_per_particle;  /* GC with DOPRI5 in FP64: 4 (default, 0) spreads one
+                                    guiding centre over a lane quad, 2 over a lane pair;
+                                    1 = one lane per particle.  Results do not depend
+                                    on it */
     int32_t work_order;          /* 0: particles with the shortest output step (slot
                                     RAPT_S_DT at upload) first; 1: input order.  Results
                                     per particle do not depend on it */
@@ -236,6 +238,12 @@ int rapt_batch_destroy(rapt_batch *b);
 /* state: RAPT_NSLOT x n slot-major host array; flags: per-particle int32
  * (bit 0: advance already started; bits 1-2: adaptive segment mode). */
 int rapt_batch_upload(rapt_batch *b, const double *state, const int32_t *flags);
+/* Work order by cost: with work_order 0, runs take particles in descending
+ * cost[i] (e.g. each particle's accepted steps in an earlier or trial
+ * launch) instead of by output step.  The next upload restores the output-
+ * step order.  No reference counterpart: scheduling of the batched tracer.
+ * Results per particle do not depend on the order. */
+int rapt_batch_set_cost(rapt_batch *b, const double *cost);
 /* Restore the state and flags of the last upload (device to device). */
 int rapt_batch_reset(rapt_batch *b);
 /* Row storage: row_cap[i] rows of `width` doubles for particle i (width 7

@@ -109,6 +109,7 @@ def lib():
     L.rapt_batch_upload.argtypes = [vp, dp, i32p]
     L.rapt_batch_set_rows.argtypes = [vp, C.c_int32, i64p]
     L.rapt_batch_reset.argtypes = [vp]
+    L.rapt_batch_set_cost.argtypes = [vp, dp]
     L.rapt_batch_run.argtypes = [vp, C.POINTER(TraceDesc), C.POINTER(Counters)]
     L.rapt_batch_download.argtypes = [vp, dp, i32p, i32p, i64p, dp, i64p]
     L.rapt_trace.argtypes = [vp, C.POINTER(TraceDesc), C.c_int64, dp, i32p, i64p, dp, i64p,
@@ -121,7 +122,7 @@ def lib():
                "rapt_mlp_forward", "rapt_mlp_infer",
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
-               "rapt_batch_reset"):
+               "rapt_batch_reset", "rapt_batch_set_cost"):
         getattr(L, nm).restype = C.c_int
     _lib = L
     return L
@@ -294,6 +295,14 @@ class Batch:
                     break
         finally:
             lib().rapt_batch_epochs_end(self.h)
+
+    def set_cost(self, cost):
+        """Take particles in descending cost (e.g. accepted steps of a trial
+        launch) instead of by output step; the next upload restores the
+        output-step order."""
+        cost = np.ascontiguousarray(cost, dtype=np.float64)
+        assert cost.shape == (self.n,)
+        self.ctx.check(lib().rapt_batch_set_cost(self.h, _dp(cost)), "rapt_batch_set_cost")
 
     def reset(self):
         """Restore the uploaded state on the device (no host transfer)."""

@@ -1,10 +1,10 @@
 """Launch-shape autotuner for the batched tracers.
 
-The best launch shape depends on the batch: at 65,536 guiding centres the
-four-lane kernel wins (one lane per particle leaves one wave per SIMD and the
-slowest particle's chain sets the time), at 1,048,576 the one-lane kernel
-wins (enough particles to fill the chip, and it spends fewer issue slots per
-step).  The tuner times short trial launches of the real batch -- the same
+The best launch shape depends on the batch: at 65,536 guiding centres a
+grouped kernel (two or four lanes per guiding centre) wins -- one lane per
+particle leaves one wave per SIMD and the slowest particle's chain sets the
+time -- and at 1,048,576 the one-lane kernel wins (enough particles to fill
+the chip, and it spends the fewest issue slots per step).  The tuner times short trial launches of the real batch -- the same
 particles, the same work order, a fraction of the advance -- for each
 candidate (block size, lanes per particle, resident lanes, i.e. the work
 refill threshold) and keeps the one with the highest accepted-steps rate.
@@ -27,10 +27,11 @@ _cache = {}
 def candidates(mode, stepper, precision, n):
     """Candidate launch shapes: dicts of rapt_trace_desc fields."""
     out = []
-    quad_ok = (mode == N.MODE_GC and stepper == N.STEP_DOPRI5 and precision == N.PREC_FP64)
+    grouped_ok = (mode == N.MODE_GC and stepper == N.STEP_DOPRI5 and precision == N.PREC_FP64)
     for block in (256, 128):
-        if quad_ok:
+        if grouped_ok:
             out.append(dict(lanes_per_particle=4, block=block, lanes=0))
+            out.append(dict(lanes_per_particle=2, block=block, lanes=0))
         out.append(dict(lanes_per_particle=1, block=block, lanes=0))
     # fewer resident lanes than particles: refill from the queue sooner
     if n >= 8 * 65536:
@@ -38,17 +39,20 @@ def candidates(mode, stepper, precision, n):
     return out
 
 
-def key(desc, model, n):
+def key(desc, model, n, ordered=False):
     return (desc.mode, desc.stepper, model, desc.precision, desc.eom,
-            int(math.log2(max(n, 1))))
+            int(math.log2(max(n, 1))), ordered)
 
 
-def tune(field, S, flags, row_cap, desc, trial_fraction=0.1, device=0, force=False, log=None):
+def tune(field, S, flags, row_cap, desc, trial_fraction=0.1, device=0, force=False, log=None,
+         cost=None):
     """Choose a launch shape for this batch.  Returns (best dict, table of
-    (shape, accepted steps/s)).  desc is not modified."""
+    (shape, accepted steps/s)).  desc is not modified.  cost: the work order
+    the run will use (Batch.set_cost), applied to the trials too, so that
+    they see the run's schedule."""
     n = S.shape[1]
     fd = field.descriptor()
-    k = key(desc, fd["model"], n)
+    k = key(desc, fd["model"], n, cost is not None)
     if not force and k in _cache:
         return _cache[k], []
     ctx = N.context(device)
@@ -58,6 +62,8 @@ def tune(field, S, flags, row_cap, desc, trial_fraction=0.1, device=0, force=Fal
     table = []
     try:
         b.upload(S, flags)
+        if cost is not None:
+            b.set_cost(cost)
         b.set_rows(width, row_cap)
         trial = N.TraceDesc.from_buffer_copy(desc)
         trial.delta = desc.delta * trial_fraction
@@ -75,6 +81,27 @@ def tune(field, S, flags, row_cap, desc, trial_fraction=0.1, device=0, force=Fal
     best = max(table, key=lambda t: t[1])[0]
     _cache[k] = best
     return best, table
+
+
+def trial_cost(field, S, flags, row_cap, desc, trial_fraction=0.1, device=0):
+    """Per-particle cost estimate for the library's work order
+    (Batch.set_cost): accepted steps of each particle in a trial launch of the
+    same batch over a fraction of the advance.  Step counts per unit of
+    simulated time change slowly along a trace, so the trial ranks the
+    particles nearly as the full advance does."""
+    n = S.shape[1]
+    ctx = N.context(device)
+    ctx.set_field(field.descriptor())
+    b = N.Batch(ctx, n)
+    try:
+        b.upload(S, flags)
+        b.set_rows(N.ROW_WIDTH[desc.mode], row_cap)
+        trial = N.TraceDesc.from_buffer_copy(desc)
+        trial.delta = desc.delta * trial_fraction
+        b.run(trial)
+        return b.download(rows=False)["naccpt"].astype(np.float64)
+    finally:
+        b.close()
 
 
 def apply(desc, shape):

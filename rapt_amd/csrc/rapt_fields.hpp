@@ -42,6 +42,10 @@ struct FieldC {
   double gstep;
   double tstep;
   double p[16];
+  // derived on the host when the field is set (IEEE divisions, so the same
+  // values the device would compute)
+  double i2d;   // 1 / (2 gstep): central-difference scale in space
+  double ihts;  // 0.5 / tstep: central-difference scale in time (0 for static fields)
 };
 
 template <class T>
@@ -212,7 +216,7 @@ template <int M>
 __device__ __forceinline__ void grad_curl(const FieldC& F, double t, double x, double y,
                                           double z, V3& gB, V3& cb) {
   const double d = F.gstep;
-  const double i2d = 1.0 / (2.0 * d);
+  const double i2d = F.i2d;
   double m[6];
   V3 u[6];
 #pragma unroll
@@ -240,7 +244,7 @@ __device__ __forceinline__ V3 dbdt(const FieldC& F, double t, double x, double y
   const double d = F.tstep;
   V3 b1 = unitb<M>(F, t - d, x, y, z);
   V3 b2 = unitb<M>(F, t + d, x, y, z);
-  const double s = 0.5 / d;
+  const double s = F.ihts;
   return V3{(b2.x - b1.x) * s, (b2.y - b1.y) * s, (b2.z - b1.z) * s};
 }
 
@@ -249,7 +253,7 @@ template <int M>
 __device__ __forceinline__ double jacobian_maxabs(const FieldC& F, double t, double x, double y,
                                                   double z) {
   const double d = F.gstep;
-  const double i2d = 1.0 / (2.0 * d);
+  const double i2d = F.i2d;
   double mx = 0.0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -327,20 +331,33 @@ __device__ __forceinline__ void rhs_lorentz(const FieldC& F, double charge, doub
 
 enum Eom : int { EOM_TCB = 0, EOM_BC = 1, EOM_NT = 2 };
 
+// Per-particle constants of the guiding-centre algebra, formed once per
+// particle (or step) instead of once per right-hand side: 1/q, 1/(m c) and
+// the Lorentz factor of the speed v (used by the BC and NT equations only).
+struct GcK {
+  double iq, imc, gamv;
+};
+__device__ __forceinline__ GcK gc_consts(int eom, double charge, double mass, double v) {
+  GcK k{rcp_nr(charge), rcp_nr(mass * C_LIGHT), 1.0};
+  if (eom != EOM_TCB) k.gamv = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
+  return k;
+}
+
 // Guiding-centre equations (rapt/GuidingCenter.py:329-395).  Y = (x, y, z, ppar).
 // The algebra, given B, grad|B|, curl b and (non-static TCB only) db/dt at the
-// guiding centre; always FP64.
+// guiding centre; always FP64.  K = gc_consts(eom, charge, mass, v).
 __device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, bool is_static,
                                            double charge, double mass, double mu, double v,
-                                           bool eq, const double* Y, const V3& B, const V3& gB,
-                                           const V3& cb, const V3& db, double* out) {
+                                           const GcK& K, bool eq, const double* Y, const V3& B,
+                                           const V3& gB, const V3& cb, const V3& db,
+                                           double* out) {
   const double ppar = Y[3];
   const double bb = dot3(B, B);
   const double ibm = rsqrt_nr(bb);
   const double Bmag = bb * ibm;
   const V3 ub{B.x * ibm, B.y * ibm, B.z * ibm};
   if (eom == EOM_NT) {
-    const double gamma = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
+    const double gamma = K.gamv;
     const double gm = gamma * mass;
     const double coef = (gm * (v * v) + (ppar * ppar) / gm) / (2.0 * charge * (Bmag * Bmag));
     const V3 c = cross3(ub, gB);
@@ -350,12 +367,12 @@ __device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, boo
     out[2] = coef * c.z + pg * ub.z;
     out[3] = -mu * dot3(ub, gB) / gamma;
   } else {
-    const double iq = rcp_nr(charge);
+    const double iq = K.iq;
     const V3 Bs{B.x + ppar * cb.x * iq, B.y + ppar * cb.y * iq, B.z + ppar * cb.z * iq};
     const double Bspar = dot3(Bs, ub);
     const double iBs = rcp_nr(Bspar);
     if (eom == EOM_BC) {
-      const double gamma = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
+      const double gamma = K.gamv;
       const double a = ppar / (gamma * mass);
       const double bcoef = mu / (charge * gamma);
       const V3 c = cross3(ub, gB);
@@ -364,7 +381,7 @@ __device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, boo
       out[2] = (a * Bs.z + bcoef * c.z) * iBs;
       out[3] = -mu * dot3(Bs, gB) / (gamma * Bspar);
     } else {
-      const double imc = rcp_nr(mass * C_LIGHT);
+      const double imc = K.imc;
       const double pr = ppar * imc;
       const double gamma = __builtin_sqrt(1.0 + 2.0 * mu * Bmag * (imc * (1.0 / C_LIGHT)) + pr * pr);
       const double igam = rcp_nr(gamma);
@@ -396,14 +413,14 @@ __device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, boo
 
 template <int M>
 __device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, double mass,
-                                       double mu, double v, bool eq, double t, const double* Y,
-                                       double* out) {
+                                       double mu, double v, const GcK& K, bool eq, double t,
+                                       const double* Y, double* out) {
   const V3 B = fieldB<M>(F, t, Y[0], Y[1], Y[2]);
   V3 gB, cb, db{0.0, 0.0, 0.0};
   grad_curl<M>(F, t, Y[0], Y[1], Y[2], gB, cb);
   if (eom == EOM_TCB && !F.is_static) db = dbdt<M>(F, t, Y[0], Y[1], Y[2]);
-  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, eq, Y, B,
-             gB, cb, db, out);
+  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, K, eq, Y,
+             B, gB, cb, db, out);
 }
 
 // ---- FP32-field precision mode ----------------------------------------------------
@@ -463,12 +480,12 @@ __device__ __forceinline__ void gc_fields_f32(const FieldC& F, double t, double 
 
 template <int M>
 __device__ __forceinline__ void rhs_gc_f32(const FieldC& F, int eom, double charge, double mass,
-                                           double mu, double v, bool eq, double t,
+                                           double mu, double v, const GcK& K, bool eq, double t,
                                            const double* Y, double* out) {
   V3 B, gB, cb, db;
   gc_fields_f32<M>(F, t, Y[0], Y[1], Y[2], B, gB, cb, db, eom == EOM_TCB && !F.is_static);
-  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, eq, Y, B,
-             gB, cb, db, out);
+  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, K, eq, Y,
+             B, gB, cb, db, out);
 }
 
 // Lorentz force with the field and the force in float; gamma*m in double.

@@ -59,11 +59,12 @@ struct GcRHS {
   double charge, mass, mu, v;
   bool eq;
   int64_t nf;
+  GcK K;  // gc_consts(eom, charge, mass, v), formed once per step
   __device__ __forceinline__ void operator()(double t, const double* y, double* o) {
     if constexpr (P == RAPT_PREC_FP32_FIELD) {
-      rhs_gc_f32<M>(F, eom, charge, mass, mu, v, eq, t, y, o);
+      rhs_gc_f32<M>(F, eom, charge, mass, mu, v, K, eq, t, y, o);
     } else {
-      rhs_gc<M>(F, eom, charge, mass, mu, v, eq, t, y, o);
+      rhs_gc<M>(F, eom, charge, mass, mu, v, K, eq, t, y, o);
     }
     ++nf;
   }
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0};
+        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v)};
         if (stepper != RAPT_STEP_RKF45) f(x, y, k1);
         c_fcn += f.nf;
       }
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0};
+        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v)};
         const double at = (KIND == KIND_ADAPTIVE) ? A.gc_atol : A.atol;
         const double rt = (KIND == KIND_ADAPTIVE) ? A.gc_rtol : A.rtol;
         h = hinit<4, P == RAPT_PREC_FP32_FIELD>(f, x, y, k1, stepper == RAPT_STEP_DOP853 ? 8 : 5, hmax, at, rt);
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0};
+        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v)};
         const double at = (KIND == KIND_ADAPTIVE) ? A.gc_atol : A.atol;
         const double rt = (KIND == KIND_ADAPTIVE) ? A.gc_rtol : A.rtol;
         if (KIND == KIND_ADAPTIVE || stepper == RAPT_STEP_DOPRI5) {
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(256) void k_field_eval(FieldC F, int what, int64_t 
       break;
     }
     case 5: {
-      const double d = F.gstep, i2d = 1.0 / (2.0 * d);
+      const double d = F.gstep, i2d = F.i2d;
       for (int k = 0; k < 3; ++k) {
         V3 Bp = fieldB<M>(F, t, x + (k == 0 ? d : 0.0), y + (k == 1 ? d : 0.0), z + (k == 2 ? d : 0.0));
         V3 Bm = fieldB<M>(F, t, x - (k == 0 ? d : 0.0), y - (k == 1 ? d : 0.0), z - (k == 2 ? d : 0.0));

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdarg.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -112,7 +113,8 @@ static CtlParams ctl(double safety, double dfactor, double ifactor, double beta,
   return p;
 }
 
-void rapt_launch_gc4(hipStream_t st, int model, const KArgs& A, int block, int64_t groups);
+void rapt_launch_gcn(hipStream_t st, int model, int lanes_per_particle, const KArgs& A,
+                     int block, int64_t groups);
 
 void rapt_launch_trace_f32(hipStream_t st, int model, int kind, const KArgs& A, int block,
                            int64_t lanes);
@@ -177,8 +179,10 @@ int rapt_field_set(rapt_ctx* c, const rapt_field_desc* f) {
   if (!(f->gstep > 0.0)) return set_err(c, RAPT_E_ARG, "gradientstepsize must be > 0");
   if (!f->is_static && !(f->tstep > 0.0))
     return set_err(c, RAPT_E_ARG, "timederivstepsize must be > 0");
-  static_assert(sizeof(FieldC) == sizeof(rapt_field_desc), "descriptor layout");
-  memcpy(&c->field, f, sizeof(FieldC));
+  static_assert(offsetof(FieldC, i2d) == sizeof(rapt_field_desc), "descriptor layout");
+  memcpy(&c->field, f, sizeof(rapt_field_desc));
+  c->field.i2d = 1.0 / (2.0 * f->gstep);
+  c->field.ihts = f->is_static ? 0.0 : 0.5 / f->tstep;
   c->field_set = true;
   return RAPT_OK;
 }
@@ -303,6 +307,23 @@ int rapt_batch_upload(rapt_batch* b, const double* state, const int32_t* flags) 
   return RAPT_OK;
 }
 
+int rapt_batch_set_cost(rapt_batch* b, const double* cost) {
+  if (!b || !cost) return RAPT_E_ARG;
+  rapt_ctx* c = b->ctx;
+  HIP_OK(c, hipSetDevice(c->device));
+  // Longest-processing-time-first: the most expensive traces start first,
+  // cheap ones fill the queue's end, and traces of similar cost share a
+  // wavefront.  Ties keep the input order.
+  std::vector<int64_t> order((size_t)b->n);
+  for (int64_t i = 0; i < b->n; ++i) order[(size_t)i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [cost](int64_t a, int64_t q) { return cost[a] > cost[q]; });
+  HIP_OK(c, hipMemcpyAsync(b->d_order, order.data(), sizeof(int64_t) * b->n,
+                           hipMemcpyHostToDevice, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  return RAPT_OK;
+}
+
 int rapt_batch_reset(rapt_batch* b) {
   if (!b) return RAPT_E_ARG;
   rapt_ctx* c = b->ctx;
@@ -355,7 +376,7 @@ int rapt_batch_set_rows(rapt_batch* b, int32_t width, const int64_t* row_cap) {
 
 // Validates a descriptor and builds the kernel arguments of a batch run.
 static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& block_out,
-                       int64_t& lanes_out, bool& quad_out) {
+                       int64_t& lanes_out, bool& grouped_out) {
   if (!b || !d) return RAPT_E_ARG;
   rapt_ctx* c = b->ctx;
   if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
@@ -369,8 +390,9 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
     return set_err(c, RAPT_E_ARG, "PARTICLE mode steps with DOP853 or RKF45");
   if (d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOP853)
     return set_err(c, RAPT_E_ARG, "GC mode steps with DOPRI5 or RKF45");
-  if (d->lanes_per_particle != 0 && d->lanes_per_particle != 1 && d->lanes_per_particle != 4)
-    return set_err(c, RAPT_E_ARG, "lanes_per_particle must be 0, 1 or 4");
+  if (d->lanes_per_particle != 0 && d->lanes_per_particle != 1 && d->lanes_per_particle != 2 &&
+      d->lanes_per_particle != 4)
+    return set_err(c, RAPT_E_ARG, "lanes_per_particle must be 0, 1, 2 or 4");
   if (d->work_order != 0 && d->work_order != 1)
     return set_err(c, RAPT_E_ARG, "work_order must be 0 or 1");
   if (d->precision != RAPT_PREC_FP64 && d->precision != RAPT_PREC_FP32_FIELD)
@@ -436,25 +458,30 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   A.rows = b->d_rows;
   A.work = b->d_work;
   A.order = d->work_order == 0 ? b->d_order : nullptr;
+  {
+    const char* e = getenv("RAPT_HOT");
+    A.hot = e ? atoll(e) : 0;
+  }
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
   if (lanes > b->n) lanes = b->n;
   if (lanes < 1) lanes = 1;
-  // the four-lane kernel is FP64 only
-  const bool quad = d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOPRI5 &&
+  // the two- and four-lane kernels are FP64 only
+  const bool grouped = d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOPRI5 &&
                     d->lanes_per_particle != 1 && d->precision == RAPT_PREC_FP64;
   lanes_out = lanes;
-  quad_out = quad;
+  grouped_out = grouped;
   return RAPT_OK;
 }
 
 // The tracer kernel of a prepared run on the context's stream.
 static void launch_tracer(rapt_batch* b, const rapt_trace_desc* d, const KArgs& A, int block,
-                          int64_t lanes, bool quad) {
+                          int64_t lanes, bool grouped) {
   rapt_ctx* c = b->ctx;
-  if (quad) {
-    int64_t groups = d->lanes > 0 ? (d->lanes + 3) / 4 : b->n;
+  if (grouped) {
+    const int L = d->lanes_per_particle == 2 ? 2 : 4;
+    int64_t groups = d->lanes > 0 ? (d->lanes + L - 1) / L : b->n;
     if (groups > b->n) groups = b->n;
-    rapt_launch_gc4(c->stream, c->field.model, A, block, groups);
+    rapt_launch_gcn(c->stream, c->field.model, L, A, block, groups);
   } else if (d->precision == RAPT_PREC_FP32_FIELD) {
     const int kind = d->mode == RAPT_MODE_PARTICLE ? KIND_PARTICLE
                                                    : (d->mode == RAPT_MODE_GC ? KIND_GC
@@ -477,12 +504,12 @@ int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) 
   KArgs A;
   int block = 256;
   int64_t lanes = 0;
-  bool quad = false;
-  const int rc = prepare_run(b, d, A, block, lanes, quad);
+  bool grouped = false;
+  const int rc = prepare_run(b, d, A, block, lanes, grouped);
   if (rc) return rc;
   HIP_OK(c, hipMemsetAsync(b->d_work, 0, sizeof(unsigned long long) * 8, c->stream));
   HIP_OK(c, hipEventRecord(c->ev0, c->stream));
-  launch_tracer(b, d, A, block, lanes, quad);
+  launch_tracer(b, d, A, block, lanes, grouped);
   HIP_OK(c, hipGetLastError());
   HIP_OK(c, hipEventRecord(c->ev1, c->stream));
   unsigned long long w[8];
@@ -609,8 +636,8 @@ int rapt_batch_epochs_begin(rapt_batch* b, const rapt_trace_desc* d, int32_t row
   KArgs A;
   int block = 256;
   int64_t lanes = 0;
-  bool quad = false;
-  int rc = prepare_run(b, d, A, block, lanes, quad);
+  bool grouped = false;
+  int rc = prepare_run(b, d, A, block, lanes, grouped);
   b->width = w0;
   if (rc) {
     rapt_batch_epochs_end(b);
@@ -625,7 +652,7 @@ int rapt_batch_epochs_begin(rapt_batch* b, const rapt_trace_desc* d, int32_t row
     return set_err(c, RAPT_E_HIP, "graph capture failed to start");
   }
   hipLaunchKernelGGL(k_epoch_prep, dim3(1), dim3(64), 0, st, b->d_work, b->d_left);
-  launch_tracer(b, d, A, block, lanes, quad);
+  launch_tracer(b, d, A, block, lanes, grouped);
   hipLaunchKernelGGL(k_epoch_left, dim3(64), dim3(256), 0, st, n, b->d_status, b->d_left);
   (void)hipMemcpyAsync(b->h_rows, b->d_erows, sizeof(double) * nr * width, hipMemcpyDeviceToHost,
                        st);

@@ -279,11 +279,13 @@ def _gc_batch(field, n, seed_pa=None):
     ("crossedEB", lambda: F.UniformCrossedEB(Ey=1e-3, Bz=1e-7), "TaoChanBrizardEOM"),
     ("dipunif", lambda: F.DipolePlusUniform(), "TaoChanBrizardEOM"),
 ])
-def test_gc_quad_lanes_match_single_lane(name, field, eom):
-    """Four lanes per guiding centre (the default) against one lane per
-    guiding centre: bit-identical rows, step counts and statuses.  The library
-    is built without FMA contraction and both kernels form every stencil
-    point, sum and norm in the same order, so any difference is a defect."""
+@pytest.mark.parametrize("lpp", [4, 2])
+def test_gc_grouped_lanes_match_single_lane(name, field, eom, lpp):
+    """Four lanes per guiding centre (the default) and two lanes per guiding
+    centre against one lane per guiding centre: bit-identical rows, step
+    counts and statuses.  The library is built without FMA contraction and
+    the kernels form every stencil point, sum and norm in the same order, so
+    any difference is a defect."""
     f = field()
     n = 40
     last, mu, v = _gc_batch(f, n)
@@ -291,7 +293,7 @@ def test_gc_quad_lanes_match_single_lane(name, field, eom):
     a = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, eom=eom,
                           lanes_per_particle=1)
     b = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, eom=eom,
-                          lanes_per_particle=4)
+                          lanes_per_particle=lpp)
     np.testing.assert_array_equal(a.status, b.status)
     np.testing.assert_array_equal(a.naccpt, b.naccpt)
     for x, y in zip(a.rows, b.rows):
@@ -300,10 +302,11 @@ def test_gc_quad_lanes_match_single_lane(name, field, eom):
     np.testing.assert_array_equal(b.state, a.state)
 
 
-def test_gc_quad_refill_and_adiabaticity_check():
-    """Quad kernel with fewer resident quads than guiding centres (work
-    refill) and the adiabaticity check on: bit-identical rows and statuses to
-    the one-lane kernel."""
+@pytest.mark.parametrize("lpp", [4, 2])
+def test_gc_grouped_refill_and_adiabaticity_check(lpp):
+    """Two- and four-lane kernels with fewer resident groups than guiding
+    centres (work refill) and the adiabaticity check on: bit-identical rows
+    and statuses to the one-lane kernel."""
     f = F.DoubleDipole()
     n = 150
     last, mu, v = _gc_batch(f, n)
@@ -311,13 +314,13 @@ def test_gc_quad_refill_and_adiabaticity_check():
     a = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, check=True,
                           lanes_per_particle=1)
     b = batch.advance_gcs(f, last, K.m_el, -K.e, mu, v, 0.05, 1.0, params=prm, check=True,
-                          lanes_per_particle=4, lanes=64)
+                          lanes_per_particle=lpp, lanes=64)
     np.testing.assert_array_equal(a.status, b.status)
     for x, y in zip(a.rows, b.rows):
         np.testing.assert_array_equal(y, x)
 
 
-@pytest.mark.parametrize("lpp", [1, 4])
+@pytest.mark.parametrize("lpp", [1, 2, 4])
 def test_work_order_does_not_change_results(lpp):
     """The library's work order (shortest output step first, the default)
     and input order give bit-identical per-particle results."""
@@ -416,7 +419,7 @@ def test_autotuner_picks_a_candidate_and_keeps_results():
         np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("lpp", [4, 1])
+@pytest.mark.parametrize("lpp", [4, 2, 1])
 def test_output_epochs_reproduce_one_launch(lpp):
     """A run in hipGraph-replayed output epochs of 7 rows per particle gives
     each particle's rows of one uninterrupted launch, epoch by epoch (every
@@ -450,6 +453,44 @@ def test_output_epochs_reproduce_one_launch(lpp):
     for i in range(n):
         np.testing.assert_array_equal(np.vstack(got[i]), one.rows[i])
     np.testing.assert_array_equal(status, one.status)
+
+
+@pytest.mark.parametrize("lpp", [4, 2, 1])
+def test_cost_work_order_does_not_change_results(lpp):
+    """A work order set by cost (Batch.set_cost: here a trial launch's step
+    counts and a random permutation, with refill) gives the rows, states and
+    statuses of the default order, bit for bit."""
+    f = F.DoubleDipole()
+    n = 96
+    last, mu, v = _gc_batch(f, n)
+    dt = np.random.default_rng(7).uniform(0.02, 0.2, n)
+    prm = P(solvertolerances=(1e-8, 1e-8))
+    S = batch.gc_state(last, K.m_el, -K.e, mu, v, dt)
+    cap = batch._rows_needed(0.6, dt)
+    desc = N.make_desc(N.MODE_GC, prm, 0.6, lanes_per_particle=lpp, lanes=48)
+    ctx = N.context()
+    ctx.set_field(f.descriptor())
+    b = N.Batch(ctx, n)
+    try:
+        b.upload(S, np.zeros(n, np.int32))
+        b.set_rows(5, cap)
+        b.run(desc)
+        ref = b.download()
+        trial = N.TraceDesc.from_buffer_copy(desc)
+        trial.delta = 0.06
+        b.reset()
+        b.run(trial)
+        costs = [b.download(rows=False)["naccpt"].astype(float),
+                 np.random.default_rng(8).permutation(n).astype(float)]
+        for cost in costs:
+            b.set_cost(cost)
+            b.reset()
+            b.run(desc)
+            got = b.download()
+            for k in ("status", "nrows", "naccpt", "state", "rows"):
+                np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    finally:
+        b.close()
 
 
 def test_work_refill_is_exact():
