@@ -287,7 +287,7 @@ void k_trace_gcn(KArgs A) {
   double h = 0, dt = 0, tend = 0, gm = 0, tcur = 0, mass = 0, charge = 0, mu = 0, v = 0;
   double xend = 0, hmax = 0, facold = 1e-4;
   int nstep = 0, nacc_call = 0;
-  bool last = false, reject = false, k1ok = false, hotp = false, prio = false;
+  bool last = false, reject = false, k1ok = false;
   int64_t nrow = 0, rowcap = 0, rowbase = 0;
   int64_t c_acc = 0, c_rej = 0, c_stp = 0, c_fcn = 0, p_acc = 0;
   const int W = 5;
@@ -335,7 +335,6 @@ void k_trace_gcn(KArgs A) {
       int64_t got = 0;
       if (q == 0) {
         got = (int64_t)atomicAdd(&A.work[0], 1ULL);
-        hotp = got < A.hot;
         if (got < n && A.order) got = A.order[got];
       }
       pi = gbll0<L>(got);
@@ -383,19 +382,6 @@ void k_trace_gcn(KArgs A) {
       }
     }
     if (__ballot(active || !exhausted) == 0) break;
-    {
-      // a wave holding one of the first `hot` work items (the most expensive
-      // traces in cost order) issues ahead of the other waves on its SIMD
-      const bool whot = __ballot(active && hotp) != 0;
-      if (whot != prio) {
-        if (whot) {
-          __builtin_amdgcn_s_setprio(2);
-        } else {
-          __builtin_amdgcn_s_setprio(0);
-        }
-        prio = whot;
-      }
-    }
 
     if (active && !k1ok) {
       k1 = rhs_gcn<M, L>(F, A.eom, charge, mass, mu, v, gk, eq, q, off, x, y, c_fcn);

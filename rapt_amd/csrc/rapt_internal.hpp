@@ -44,7 +44,6 @@ struct KArgs {
   double* rows;
   unsigned long long* work;  // [0] next work item, [1..6] counters
   const int64_t* order;      // work item -> particle index (nullptr: identity)
-  int64_t hot;               // waves holding work items < hot issue at raised priority
 };
 
 }  // namespace rapt

@@ -458,10 +458,6 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   A.rows = b->d_rows;
   A.work = b->d_work;
   A.order = d->work_order == 0 ? b->d_order : nullptr;
-  {
-    const char* e = getenv("RAPT_HOT");
-    A.hot = e ? atoll(e) : 0;
-  }
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
   if (lanes > b->n) lanes = b->n;
   if (lanes < 1) lanes = 1;
