@@ -1,8 +1,8 @@
 // rapt_tracer.hip -- the C-ABI of librapt_hip.so (include/rapt_hip.h): contexts,
 // field evaluation, batches, and the launch of the FP64 tracer kernels.  The
 // kernel itself is k_trace in rapt_trace_kernel.hpp; the FP32-field
-// instantiations live in rapt_tracer_f32.hip and the four-lane guiding-centre
-// kernel in rapt_gc4.hip.
+// instantiations live in rapt_tracer_f32.hip and the two- and four-lane
+// guiding-centre kernels in rapt_gcn.hip.
 #include <hip/hip_runtime.h>
 
 #include <stdarg.h>
