@@ -159,6 +159,17 @@ int rapt_field_eval(rapt_ctx *ctx, int what, int64_t n, const double *tpos, doub
 int rapt_halfbounce(rapt_ctx *ctx, int64_t n, const double *tpos, const double *Bm,
                     double fieldlineresolution, double *sb, int32_t *status);
 
+/* Second invariant I = int sqrt(1 - B/Bm) ds between the mirror points for n
+ * guiding centres (flutils.eye, rapt/flutils.py:65-151, as
+ * GuidingCenter.geteye calls it once per row, rapt/GuidingCenter.py:608-624):
+ * the field-line trace, spline and mirror points of rapt_halfbounce, the
+ * integral by the same end-point substitution and Gauss-Legendre panels (the
+ * reference: Simpson on the trace points below 70 deg equatorial pitch,
+ * QUADPACK epsrel 1e-4 above).  0 where min |B| on the line reaches Bm.
+ * Arguments and status as rapt_halfbounce. */
+int rapt_eye(rapt_ctx *ctx, int64_t n, const double *tpos, const double *Bm,
+             double fieldlineresolution, double *eye, int32_t *status);
+
 /* Surrogate labels, one bounce-averaged drift evaluation per query: replaces
  * BounceCenter(pos, v, 0, pa, mass, charge, field) followed by the drift of
  * its deriv (rapt/BounceCenter.py:74-115, 235-244), flutils.bounceperiod
@@ -189,6 +200,20 @@ int rapt_bounce_center_advance(rapt_ctx *ctx, int64_t n, double *tpos, const dou
                                int64_t *naccpt, double rtol, double atol,
                                double eyegradientstep, double fieldlineresolution,
                                int32_t nsteps);
+
+/* Guiding centre and first invariant of every row of Particle trajectories:
+ * Particle.guidingcenter() and Particle.mu() (rapt/Particle.py:463-482), i.e.
+ * utils.guidingcenter (rapt/utils.py:251-326) and utils.magnetic_moment
+ * (rapt/utils.py:189-216) per row.  rows [n][7] = (t, x, y, z, px, py, pz);
+ * mass, charge: per row (per_row != 0) or one value each; tol, maxiter: the
+ * fixed-point iteration's (reference defaults 1e-3, 20).  out [n][6] = (X,
+ * Y, Z, v_par, speed, mu); status[i] = iterations used, or RAPT_GC_NOCONV
+ * (out NaN) where maxiter iterations do not reach tol (the reference prints
+ * a message and returns None). */
+#define RAPT_GC_NOCONV (-30)
+int rapt_particle_gc(rapt_ctx *ctx, int64_t n, const double *rows, const double *mass,
+                     const double *charge, int32_t per_row, double tol, int32_t maxiter,
+                     double *out, int32_t *status);
 
 /* ---- bounce-drift surrogate (no counterpart in the reference) ------------------
  * An ensemble of MLPs 3 -> 256 -> 256 -> 256 -> 3 (SiLU; BF16 weights, FP32

@@ -261,8 +261,16 @@ class GuidingCenter:
         return flutils.bounceperiod(tpos, self.field, Bmirror, v)
 
     def geteye(self, step=1):
-        from . import flutils
+        """(time, second invariant) pairs along the trajectory, every `step` rows
+        (rapt/GuidingCenter.py:608-624); I on the device (rapt_eye)."""
+        from . import _native as N
+        from . import params
         Bm = self.getBm()
         rows = self.trajectory[::step]
-        res = np.array([flutils.eye(r[:4], self.field, b) for r, b in zip(rows, Bm[::step])])
+        res, st = N.context().eye(self.field.descriptor(), rows[:, :4], Bm[::step],
+                                  params["fieldlineresolution"])
+        bad = np.nonzero(st != 1)[0]
+        if bad.size:
+            raise RuntimeError("geteye: field-line integral failed at row %d (status %d)"
+                               % (bad[0] * step, st[bad[0]]))
         return np.array(list(zip(rows[:, 0], res)))

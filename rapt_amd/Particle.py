@@ -210,23 +210,25 @@ class Particle:
         return N.context().field_eval(self.field.descriptor(), "magB",
                                       self.trajectory[:, :4])[:, 0]
 
+    def _gc_rows(self):
+        """Guiding centre, v_par, speed and mu of every row, on the device
+        (rapt_particle_gc: utils.guidingcenter and utils.magnetic_moment per row)."""
+        from . import _native as N
+        out, st = N.context().particle_gc(self.field.descriptor(), self.trajectory, self.mass,
+                                          self.charge)
+        bad = np.nonzero(st == N.GC_NOCONV)[0]
+        if bad.size:
+            raise RuntimeError("guidingcenter: no convergence to 1e-3 in 20 iterations at "
+                               "row %d" % bad[0])
+        return out
+
     def guidingcenter(self):
-        out = []
-        for row in self.trajectory:
-            t, r, mom = row[0], row[1:4], row[4:]
-            gm = np.sqrt(self.mass ** 2 + np.dot(mom, mom) / c ** 2)
-            rgc, vp, spd = ru.guidingcenter(t, r, mom / gm, self.field, self.mass, self.charge)
-            out.append(list(rgc) + [vp, spd])
-        return np.array(out)
+        """(X, Y, Z, v_par, speed) of every row (rapt/Particle.py:463-472)."""
+        return self._gc_rows()[:, :5]
 
     def mu(self):
-        out = []
-        for row in self.trajectory:
-            t, r, mom = row[0], row[1:4], row[4:]
-            gm = np.sqrt(self.mass ** 2 + np.dot(mom, mom) / c ** 2)
-            rgc, vp, spd = ru.guidingcenter(t, r, mom / gm, self.field, self.mass, self.charge)
-            out.append(ru.magnetic_moment(t, rgc, vp, spd, self.field, self.mass))
-        return np.array(out)
+        """First invariant of every row (rapt/Particle.py:473-482)."""
+        return self._gc_rows()[:, 5]
 
     def cycrad(self):
         t, r, mom = self.trajectory[-1, 0], self.trajectory[-1, 1:4], self.trajectory[-1, 4:]

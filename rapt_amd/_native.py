@@ -25,6 +25,7 @@ S_T, S_Y0, S_MASS, S_CHARGE, S_MU, S_V, S_DT, S_H, S_TEND, S_GM, S_TCUR = \
 FLAG_STARTED, FLAG_GCMODE, FLAG_AINIT, FLAG_FINAL = 1, 2, 4, 8
 ST_PENDING, ST_DONE, ST_SWITCH = 0, 1, 2
 ST_NSTEPS, ST_HSMALL, ST_NAN, ST_GCITER, ST_CAP, ST_HMIN = -2, -3, -4, -5, -6, -7
+GC_NOCONV = -30  # rapt_particle_gc: fixed-point iteration did not reach tol
 STATUS_TEXT = {1: "ok", 2: "adiabaticity switch", -2: "larger nsteps is needed",
                -3: "step size becomes too small", -4: "non-finite state",
                -5: "guiding-centre iteration did not converge", -6: "row capacity reached",
@@ -110,6 +111,9 @@ def lib():
     L.rapt_batch_set_rows.argtypes = [vp, C.c_int32, i64p]
     L.rapt_batch_reset.argtypes = [vp]
     L.rapt_batch_set_cost.argtypes = [vp, dp]
+    L.rapt_eye.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
+    L.rapt_particle_gc.argtypes = [vp, C.c_int64, dp, dp, dp, C.c_int32, C.c_double, C.c_int32,
+                                   dp, i32p]
     L.rapt_batch_run.argtypes = [vp, C.POINTER(TraceDesc), C.POINTER(Counters)]
     L.rapt_batch_download.argtypes = [vp, dp, i32p, i32p, i64p, dp, i64p]
     L.rapt_trace.argtypes = [vp, C.POINTER(TraceDesc), C.c_int64, dp, i32p, i64p, dp, i64p,
@@ -122,7 +126,7 @@ def lib():
                "rapt_mlp_forward", "rapt_mlp_infer",
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
-               "rapt_batch_reset", "rapt_batch_set_cost"):
+               "rapt_batch_reset", "rapt_batch_set_cost", "rapt_eye", "rapt_particle_gc"):
         getattr(L, nm).restype = C.c_int
     _lib = L
     return L
@@ -197,6 +201,38 @@ class Context:
                    "rapt_halfbounce")
         return sb, st
 
+
+    def eye(self, desc, tpos, Bm, fieldlineresolution):
+        """Second invariant I per guiding centre on the device; returns (I, status)."""
+        self.set_field(desc)
+        tpos = np.ascontiguousarray(np.atleast_2d(tpos), dtype=np.float64)
+        n = tpos.shape[0]
+        Bm = np.ascontiguousarray(np.broadcast_to(Bm, (n,)), dtype=np.float64)
+        out = np.zeros(n)
+        st = np.zeros(n, np.int32)
+        self.check(lib().rapt_eye(self.h, n, _dp(tpos), _dp(Bm), float(fieldlineresolution),
+                                  _dp(out), _ip32(st)), "rapt_eye")
+        return out, st
+
+    def particle_gc(self, desc, rows, mass, charge, tol=1e-3, maxiter=20):
+        """Guiding centre and mu of every Particle row (t, x, y, z, px, py, pz) on the
+        device; mass and charge scalars or per row.  Returns (out (n, 6) = X, Y, Z,
+        v_par, speed, mu; status = iterations used, or GC_NOCONV)."""
+        self.set_field(desc)
+        rows = np.ascontiguousarray(np.atleast_2d(rows), dtype=np.float64)
+        n = rows.shape[0]
+        assert rows.shape[1] == 7
+        per_row = np.ndim(mass) > 0 or np.ndim(charge) > 0
+        m = np.ascontiguousarray(np.broadcast_to(mass, (n,)) if per_row else [mass],
+                                 dtype=np.float64)
+        q = np.ascontiguousarray(np.broadcast_to(charge, (n,)) if per_row else [charge],
+                                 dtype=np.float64)
+        out = np.zeros((n, 6))
+        st = np.zeros(n, np.int32)
+        self.check(lib().rapt_particle_gc(self.h, n, _dp(rows), _dp(m), _dp(q), int(per_row),
+                                          float(tol), int(maxiter), _dp(out), _ip32(st)),
+                   "rapt_particle_gc")
+        return out, st
 
     def bounce_labels(self, desc, query, mass, charge, eyegradientstep, fieldlineresolution):
         """(T_b, omega_d, mirror latitude) per (L, alpha_eq [deg], log10 K [eV]) query;

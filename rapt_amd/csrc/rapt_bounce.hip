@@ -52,7 +52,7 @@ struct BounceArgs {
   const double* Bm;
   double res;
   int32_t cap;   // points per half line
-  int32_t pad;
+  int32_t what;  // k_halfbounce output: 0 S_b, 1 the second invariant I
   int64_t lanes; // scratch stride
   double* S;     // [2*cap][lanes] arc length
   double* Bv;    // [2*cap][lanes] |B|
@@ -362,11 +362,11 @@ __global__ __launch_bounds__(256) void k_halfbounce(BounceArgs A) {
   const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= A.lanes) return;
   for (int64_t i = lane; i < A.n; i += A.lanes) {
-    double sb;
+    double sb, ic = 0.0;
     const int32_t st = line_integrals<M>(A, lane, A.tpos[4 * i], A.tpos[4 * i + 1],
                                          A.tpos[4 * i + 2], A.tpos[4 * i + 3], A.Bm[i], &sb,
-                                         nullptr, nullptr);
-    A.out[i] = sb;
+                                         A.what ? &ic : nullptr, nullptr);
+    A.out[i] = A.what ? ic : sb;
     A.status[i] = st;
   }
 }
@@ -630,7 +630,7 @@ static void launch_lab(rapt_ctx* c, const BounceArgs& A, const LabelArgs& L) {
 // one launch, results back.  in_w doubles in per query, out_w doubles out.
 static int run_bounce(rapt_ctx* c, int64_t n, const double* in, int in_w, const double* Bm,
                       double res, bool positions, const LabelArgs* lab, double* out,
-                      int out_w, int32_t* status) {
+                      int out_w, int32_t* status, int32_t what = 0) {
   HIP_OK(c, hipSetDevice(c->device));
   BounceArgs A;
   memset(&A, 0, sizeof A);
@@ -638,6 +638,7 @@ static int run_bounce(rapt_ctx* c, int64_t n, const double* in, int in_w, const 
   A.n = n;
   A.res = res;
   A.cap = 2048;
+  A.what = what;
   A.lanes = n < 16384 ? n : 16384;
   const size_t npts = (size_t)2 * A.cap * A.lanes;
   const int narr = positions ? 6 : 4;
@@ -703,6 +704,16 @@ extern "C" int rapt_halfbounce(rapt_ctx* c, int64_t n, const double* tpos, const
     return set_err(c, RAPT_E_ARG, "fieldlineresolution must be > 0");
   if (n == 0) return RAPT_OK;
   return run_bounce(c, n, tpos, 4, Bm, fieldlineresolution, false, nullptr, sb, 1, status);
+}
+
+extern "C" int rapt_eye(rapt_ctx* c, int64_t n, const double* tpos, const double* Bm,
+                        double fieldlineresolution, double* eye, int32_t* status) {
+  if (!c || n < 0 || (n && (!tpos || !Bm || !eye || !status))) return RAPT_E_ARG;
+  if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
+  if (!(fieldlineresolution > 0.0))
+    return set_err(c, RAPT_E_ARG, "fieldlineresolution must be > 0");
+  if (n == 0) return RAPT_OK;
+  return run_bounce(c, n, tpos, 4, Bm, fieldlineresolution, false, nullptr, eye, 1, status, 1);
 }
 
 extern "C" int rapt_bounce_labels(rapt_ctx* c, int64_t n, const double* query, double mass,
