@@ -42,7 +42,9 @@ enum rapt_field_model {
                                                                    rapt/fields.py:472-511 */
     RAPT_FIELD_CURRENT_SHEET = 7,      /* B = (p0 tanh(z/p1), 0, p2), E = (p3, p4, p5) */
     RAPT_FIELD_DIPOLE_UNIFORM = 8,     /* EarthDipole(p0) + (p1, p2, p3)                */
-    RAPT_FIELD_NMODELS = 9
+    RAPT_FIELD_GRID = 9,               /* B, E sampled on a grid (rapt_grid_set)
+                                                                   rapt/fields.py:513-814 */
+    RAPT_FIELD_NMODELS = 10
 };
 
 /* Field descriptor: replaces the _Field plugin object (rapt/fields.py:5-41). */
@@ -143,6 +145,21 @@ int rapt_ctx_destroy(rapt_ctx *ctx);
 const char *rapt_last_error(rapt_ctx *ctx);
 /* replaces the field object handed to the tracer constructors */
 int rapt_field_set(rapt_ctx *ctx, const rapt_field_desc *field);
+
+/* Gridded field for RAPT_FIELD_GRID: replaces Grid's parsed files and its
+ * RegularGridInterpolator (rapt/fields.py:513-814).  Axes x[nx], y[ny],
+ * z[nz] strictly increasing (uniform spacing not required), t[nt] the
+ * snapshot times (ignored when nt = 1: a static 3-D grid).  B, E: [nt][nx]
+ * [ny][nz][3] in SI units, C order (E may be NULL: E = 0).  The library
+ * copies everything into device memory owned by the context and keeps all
+ * snapshots resident (the reference keeps a window of three).  Evaluation is
+ * trilinear in space and linear in time between the bracketing snapshots;
+ * outside the grid B and E are NaN (the reference raises ValueError), which
+ * stops a traced particle with a non-finite or step-size status.  Then
+ * rapt_field_set with model RAPT_FIELD_GRID (params unused) selects it. */
+int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt, const double *x,
+                  const double *y, const double *z, const double *t, const double *B,
+                  const double *E);
 
 /* Batched field evaluation on the device (the _Field methods,
  * rapt/fields.py:76-280).  what: 0 B, 1 E, 2 unitb, 3 magB, 4 gradB,

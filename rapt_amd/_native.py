@@ -111,6 +111,8 @@ def lib():
     L.rapt_batch_set_rows.argtypes = [vp, C.c_int32, i64p]
     L.rapt_batch_reset.argtypes = [vp]
     L.rapt_batch_set_cost.argtypes = [vp, dp]
+    L.rapt_grid_set.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp, dp, dp,
+                                dp, dp]
     L.rapt_eye.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
     L.rapt_particle_gc.argtypes = [vp, C.c_int64, dp, dp, dp, C.c_int32, C.c_double, C.c_int32,
                                    dp, i32p]
@@ -126,7 +128,8 @@ def lib():
                "rapt_mlp_forward", "rapt_mlp_infer",
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
-               "rapt_batch_reset", "rapt_batch_set_cost", "rapt_eye", "rapt_particle_gc"):
+               "rapt_batch_reset", "rapt_batch_set_cost", "rapt_eye", "rapt_particle_gc",
+               "rapt_grid_set"):
         getattr(L, nm).restype = C.c_int
     _lib = L
     return L
@@ -157,6 +160,7 @@ class Context:
         self.h = h
         self.device = device
         self._field_key = None
+        self._grid_id = 0
 
     def check(self, rc, what):
         if rc != 0:
@@ -164,9 +168,18 @@ class Context:
             raise RuntimeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
 
     def set_field(self, desc):
-        key = (desc["model"], desc["static"], desc["gstep"], desc["tstep"], desc["params"])
+        grid = desc.get("grid")
+        key = (desc["model"], desc["static"], desc["gstep"], desc["tstep"], desc["params"],
+               grid._grid_id if grid is not None else 0)
         if key == self._field_key:
             return
+        if grid is not None and self._grid_id != grid._grid_id:
+            x, y, z, t, B, E = (np.ascontiguousarray(a, dtype=np.float64)
+                                for a in grid.grid_arrays())
+            nt = B.shape[0]
+            self.check(lib().rapt_grid_set(self.h, len(x), len(y), len(z), nt, _dp(x), _dp(y),
+                                           _dp(z), _dp(t), _dp(B), _dp(E)), "rapt_grid_set")
+            self._grid_id = grid._grid_id
         f = FieldDesc()
         f.model = desc["model"]
         f.is_static = 1 if desc["static"] else 0

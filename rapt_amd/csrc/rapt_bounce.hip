@@ -673,6 +673,7 @@ static int run_bounce(rapt_ctx* c, int64_t n, const double* in, int in_w, const 
         case 2: lab ? launch_lab<2>(c, A, *lab) : launch_hb<2>(c, A); break;
         case 5: lab ? launch_lab<5>(c, A, *lab) : launch_hb<5>(c, A); break;
         case 8: lab ? launch_lab<8>(c, A, *lab) : launch_hb<8>(c, A); break;
+        case 9: lab ? launch_lab<9>(c, A, *lab) : launch_hb<9>(c, A); break;
         default:
           rc = set_err(c, RAPT_E_ARG, "field-line integrals need a dipole-type field (model %d)",
                        c->field.model);

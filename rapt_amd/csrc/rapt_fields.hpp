@@ -31,7 +31,22 @@ constexpr double TWO_PI = 6.283185307179586;
 enum FieldModel : int {
   F_ZERO = 0, F_EARTH_DIPOLE = 1, F_DOUBLE_DIPOLE = 2, F_UNIFORM_BZ = 3,
   F_UNIFORM_CROSSED_EB = 4, F_VAR_EARTH_DIPOLE = 5, F_PARABOLIC = 6,
-  F_CURRENT_SHEET = 7, F_DIPOLE_UNIFORM = 8
+  F_CURRENT_SHEET = 7, F_DIPOLE_UNIFORM = 8, F_GRID = 9
+};
+
+// A gridded field (Grid, rapt/fields.py:513-814) in HBM: B and E sampled on
+// a rectilinear grid (axes need not be uniform) at nt >= 1 times.  Values
+// are [nt][nx][ny][nz][4] doubles (x, y, z, unused), so the two z-neighbours
+// of a cell corner are one 64-byte run.  All snapshots stay resident; the
+// reference's window of three snapshots is the special case nt <= 3.
+struct GridC {
+  const double* ax[4];  // axis coordinates: x, y, z, t (t unused when nt = 1)
+  const double* B;      // [nt][nx][ny][nz][4]
+  const double* E;      // same layout, or nullptr (E = 0)
+  int32_t n[4];         // nx, ny, nz, nt
+  int32_t uniform;      // bit k: axis k is uniformly spaced
+  int32_t pad;
+  double a0[4], ida[4]; // uniform axes: first coordinate, 1 / spacing
 };
 
 // Field constants: passed by value as a kernel argument, so every lane reads
@@ -46,6 +61,7 @@ struct FieldC {
   // values the device would compute)
   double i2d;   // 1 / (2 gstep): central-difference scale in space
   double ihts;  // 0.5 / tstep: central-difference scale in time (0 for static fields)
+  GridC grid;   // F_GRID only
 };
 
 template <class T>
@@ -115,6 +131,78 @@ __device__ __forceinline__ V3T<T> cross3(const V3T<T>& a, const V3T<T>& b) {
   return V3T<T>{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// ---- gridded fields ---------------------------------------------------------
+// Cell of coordinate x on axis k: i in [0, n-2] with g[i] <= x <= g[i+1] and
+// the fraction f = (x - g[i]) / (g[i+1] - g[i]), or -1 outside [g[0], g[n-1]]
+// (scipy's RegularGridInterpolator, which the reference's Grid uses, raises
+// ValueError there).  Uniform axes take the cell from the spacing and fix it
+// up by one against the stored coordinates; others bisect.
+__device__ __forceinline__ int grid_cell(const GridC& G, int k, double x, double& f) {
+  const double* g = G.ax[k];
+  const int n = G.n[k];
+  if (!(x >= g[0] && x <= g[n - 1])) return -1;
+  int i;
+  if (G.uniform & (1 << k)) {
+    i = (int)((x - G.a0[k]) * G.ida[k]);
+    i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+    if (x < g[i]) --i;
+    else if (x > g[i + 1]) ++i;
+  } else {
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (g[mid] <= x) lo = mid; else hi = mid;
+    }
+    i = lo > n - 2 ? n - 2 : lo;
+  }
+  f = (x - g[i]) / (g[i + 1] - g[i]);
+  return i;
+}
+
+__device__ __forceinline__ double lerp(double a, double b, double f) { return a + f * (b - a); }
+
+// Multilinear interpolation of the [nt][nx][ny][nz][4] array V at (t, x, y,
+// z): trilinear in space, linear in time between the two bracketing
+// snapshots (nt > 1).  NaN outside the grid.
+__device__ __forceinline__ V3 grid_interp(const GridC& G, const double* V, double t, double x,
+                                          double y, double z) {
+  double fx, fy, fz, ft = 0.0;
+  const int i = grid_cell(G, 0, x, fx), j = grid_cell(G, 1, y, fy), k = grid_cell(G, 2, z, fz);
+  int it = 0;
+  if (G.n[3] > 1) it = grid_cell(G, 3, t, ft);
+  if ((i | j | k | it) < 0) {
+    const double nan = __builtin_nan("");
+    return V3{nan, nan, nan};
+  }
+  const int64_t sz = 4, sy = (int64_t)G.n[2] * sz, sx = (int64_t)G.n[1] * sy,
+                st = (int64_t)G.n[0] * sx;
+  double r[2][3];
+  const int nts = G.n[3] > 1 ? 2 : 1;
+  for (int a = 0; a < nts; ++a) {
+    const double* c = V + (it + a) * st + i * sx + j * sy + k * sz;
+    double e[2][3];
+#pragma unroll
+    for (int dx = 0; dx < 2; ++dx) {
+      double q[2][3];
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy) {
+        const double* p = c + dx * sx + dy * sy;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) q[dy][m] = lerp(p[m], p[sz + m], fz);
+      }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) e[dx][m] = lerp(q[0][m], q[1][m], fy);
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) r[a][m] = lerp(e[0][m], e[1][m], fx);
+  }
+  if (nts == 2) {
+    return V3{lerp(r[0][0], r[1][0], ft), lerp(r[0][1], r[1][1], ft),
+              lerp(r[0][2], r[1][2], ft)};
+  }
+  return V3{r[0][0], r[0][1], r[0][2]};
+}
+
 // ---- B(t, r) ---------------------------------------------------------------
 // T = double is the tracer's arithmetic; T = float serves the FP32-field
 // precision mode (constants rounded once to float).
@@ -169,14 +257,22 @@ __device__ __forceinline__ V3T<T> fieldB(const FieldC& F, T t, T x, T y, T z) {
     return V{bx, T(0.0), T(F.p[1])};
   } else if constexpr (M == F_CURRENT_SHEET) {
     return V{T(F.p[0]) * tanh(z / T(F.p[1])), T(0.0), T(F.p[2])};
+  } else if constexpr (M == F_GRID) {
+    static_assert(sizeof(T) == sizeof(double), "gridded fields are FP64");
+    return grid_interp(F.grid, F.grid.B, t, x, y, z);
   } else {
     return V{T(0.0), T(0.0), T(0.0)};
   }
 }
 
+// E(t, r); the analytic models' E is uniform
 template <int M, class T = double>
-__device__ __forceinline__ V3T<T> fieldE(const FieldC& F) {
-  if constexpr (M == F_UNIFORM_CROSSED_EB) {
+__device__ __forceinline__ V3T<T> fieldE(const FieldC& F, T t, T x, T y, T z) {
+  if constexpr (M == F_GRID) {
+    static_assert(sizeof(T) == sizeof(double), "gridded fields are FP64");
+    if (!F.grid.E) return V3T<T>{T(0.0), T(0.0), T(0.0)};
+    return grid_interp(F.grid, F.grid.E, t, x, y, z);
+  } else if constexpr (M == F_UNIFORM_CROSSED_EB) {
     return V3T<T>{T(0.0), T(F.p[1]), T(0.0)};
   } else if constexpr (M == F_CURRENT_SHEET) {
     return V3T<T>{T(F.p[3]), T(F.p[4]), T(F.p[5])};
@@ -187,7 +283,7 @@ __device__ __forceinline__ V3T<T> fieldE(const FieldC& F) {
 
 template <int M>
 constexpr bool has_E() {
-  return M == F_UNIFORM_CROSSED_EB || M == F_CURRENT_SHEET;
+  return M == F_UNIFORM_CROSSED_EB || M == F_CURRENT_SHEET || M == F_GRID;
 }
 
 template <int M>
@@ -314,7 +410,7 @@ __device__ __forceinline__ void rhs_lorentz(const FieldC& F, double charge, doub
   const V3 B = fieldB<M>(F, t, Y[0], Y[1], Y[2]);
   const V3 c = cross3(p, B);
   if constexpr (has_E<M>()) {
-    const V3 E = fieldE<M>(F);
+    const V3 E = fieldE<M>(F, t, Y[0], Y[1], Y[2]);
     out[3] = charge * (E.x + c.x * ig);
     out[4] = charge * (E.y + c.y * ig);
     out[5] = charge * (E.z + c.z * ig);
@@ -419,7 +515,8 @@ __device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, 
   V3 gB, cb, db{0.0, 0.0, 0.0};
   grad_curl<M>(F, t, Y[0], Y[1], Y[2], gB, cb);
   if (eom == EOM_TCB && !F.is_static) db = dbdt<M>(F, t, Y[0], Y[1], Y[2]);
-  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, K, eq, Y,
+  const V3 Ec = fieldE<M>(F, t, Y[0], Y[1], Y[2]);
+  gc_algebra(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y,
              B, gB, cb, db, out);
 }
 
@@ -484,7 +581,8 @@ __device__ __forceinline__ void rhs_gc_f32(const FieldC& F, int eom, double char
                                            const double* Y, double* out) {
   V3 B, gB, cb, db;
   gc_fields_f32<M>(F, t, Y[0], Y[1], Y[2], B, gB, cb, db, eom == EOM_TCB && !F.is_static);
-  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, K, eq, Y,
+  const V3 Ec = fieldE<M>(F, t, Y[0], Y[1], Y[2]);
+  gc_algebra(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y,
              B, gB, cb, db, out);
 }
 
@@ -506,7 +604,7 @@ __device__ __forceinline__ void rhs_lorentz_f32(const FieldC& F, double charge, 
   const V3f c = cross3(p, B);
   const float q = (float)charge;
   if constexpr (has_E<M>()) {
-    const V3f E = fieldE<M, float>(F);
+    const V3f E = fieldE<M, float>(F, (float)t, (float)Y[0], (float)Y[1], (float)Y[2]);
     out[3] = (double)(q * (E.x + c.x * ig));
     out[4] = (double)(q * (E.y + c.y * ig));
     out[5] = (double)(q * (E.z + c.z * ig));

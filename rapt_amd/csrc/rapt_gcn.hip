@@ -193,7 +193,8 @@ __device__ __forceinline__ CV<1> rhs_gc_quad(const FieldC& F, int eom, double ch
   }
   const double Y[4] = {Y0, Y1, Y2, Y3};
   double o[4];
-  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, K, eq, Y, B,
+  const V3 Ec = fieldE<M>(F, t, Y0, Y1, Y2);
+  gc_algebra(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y, B,
              gB, cb, db, o);
   return CV<1>{{q == 0 ? o[0] : (q == 1 ? o[1] : (q == 2 ? o[2] : o[3]))}};
 }
@@ -246,7 +247,8 @@ __device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, double ch
   }
   const double Y[4] = {Y0, Y1, Y2, Y3};
   double o[4];
-  gc_algebra(eom, has_E<M>(), fieldE<M>(F), F.is_static != 0, charge, mass, mu, v, K, eq, Y, B,
+  const V3 Ec = fieldE<M>(F, t, Y0, Y1, Y2);
+  gc_algebra(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y, B,
              gB, cb, db, o);
   return CV<2>{{l0 ? o[0] : o[2], l0 ? o[1] : o[3]}};
 }
@@ -575,6 +577,7 @@ static void launch_gcn_l(hipStream_t st, int model, const KArgs& A, int block, i
     case 6: launch_gcn_m<6, L>(st, A, block, groups); break;
     case 7: launch_gcn_m<7, L>(st, A, block, groups); break;
     case 8: launch_gcn_m<8, L>(st, A, block, groups); break;
+    case 9: launch_gcn_m<9, L>(st, A, block, groups); break;
   }
 }
 

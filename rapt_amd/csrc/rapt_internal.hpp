@@ -54,6 +54,9 @@ struct rapt_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   rapt::FieldC field{};
   bool field_set = false;
+  rapt::GridC grid{};         // uploaded by rapt_grid_set (device memory below)
+  double* grid_mem = nullptr;
+  bool grid_set = false;
   std::string err;
 };
 

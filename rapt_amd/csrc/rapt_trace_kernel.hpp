@@ -459,7 +459,7 @@ __global__ __launch_bounds__(256) void k_field_eval(FieldC F, int what, int64_t 
       break;
     }
     case 1: {
-      V3 E = fieldE<M>(F);
+      V3 E = fieldE<M>(F, t, x, y, z);
       o[0] = E.x; o[1] = E.y; o[2] = E.z;
       break;
     }

@@ -138,6 +138,7 @@ static void launch_model(rapt_ctx* c, const KArgs& A, int block, int64_t lanes) 
     case 6: launch_trace<6, K>(c, A, block, lanes); break;
     case 7: launch_trace<7, K>(c, A, block, lanes); break;
     case 8: launch_trace<8, K>(c, A, block, lanes); break;
+    case 9: launch_trace<9, K>(c, A, block, lanes); break;
   }
 }
 
@@ -166,6 +167,7 @@ int rapt_ctx_destroy(rapt_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  (void)hipFree(c->grid_mem);
   delete c;
   return RAPT_OK;
 }
@@ -179,11 +181,79 @@ int rapt_field_set(rapt_ctx* c, const rapt_field_desc* f) {
   if (!(f->gstep > 0.0)) return set_err(c, RAPT_E_ARG, "gradientstepsize must be > 0");
   if (!f->is_static && !(f->tstep > 0.0))
     return set_err(c, RAPT_E_ARG, "timederivstepsize must be > 0");
+  if (f->model == RAPT_FIELD_GRID && !c->grid_set)
+    return set_err(c, RAPT_E_STATE, "gridded field: no grid uploaded (rapt_grid_set)");
   static_assert(offsetof(FieldC, i2d) == sizeof(rapt_field_desc), "descriptor layout");
   memcpy(&c->field, f, sizeof(rapt_field_desc));
   c->field.i2d = 1.0 / (2.0 * f->gstep);
   c->field.ihts = f->is_static ? 0.0 : 0.5 / f->tstep;
+  c->field.grid = f->model == RAPT_FIELD_GRID ? c->grid : GridC{};
   c->field_set = true;
+  return RAPT_OK;
+}
+
+int rapt_grid_set(rapt_ctx* c, int32_t nx, int32_t ny, int32_t nz, int32_t nt, const double* x,
+                  const double* y, const double* z, const double* t, const double* B,
+                  const double* E) {
+  if (!c || !x || !y || !z || !B || (nt > 1 && !t)) return RAPT_E_ARG;
+  if (nx < 2 || ny < 2 || nz < 2 || nt < 1)
+    return set_err(c, RAPT_E_ARG, "grid needs at least 2 points per axis and 1 time");
+  const double* ax[4] = {x, y, z, t};
+  const int32_t n[4] = {nx, ny, nz, nt};
+  for (int k = 0; k < (nt > 1 ? 4 : 3); ++k)
+    for (int32_t i = 0; i + 1 < n[k]; ++i)
+      if (!(ax[k][i] < ax[k][i + 1]))
+        return set_err(c, RAPT_E_ARG, "grid axis %d is not strictly increasing", k);
+  HIP_OK(c, hipSetDevice(c->device));
+  const size_t npt = (size_t)nx * ny * nz * nt;
+  const size_t naxis = (size_t)nx + ny + nz + nt;
+  const size_t total = naxis + (E ? 8 : 4) * npt;
+  std::vector<double> h(total, 0.0);
+  size_t o = 0;
+  size_t off[4];
+  for (int k = 0; k < 4; ++k) {
+    off[k] = o;
+    if (k < 3 || nt > 1) memcpy(&h[o], ax[k], sizeof(double) * n[k]);
+    o += n[k];
+  }
+  const size_t offB = o, offE = o + 4 * npt;
+  for (size_t i = 0; i < npt; ++i) {
+    for (int m = 0; m < 3; ++m) {
+      h[offB + 4 * i + m] = B[3 * i + m];
+      if (E) h[offE + 4 * i + m] = E[3 * i + m];
+    }
+  }
+  double* mem = nullptr;
+  if (hipMalloc(&mem, sizeof(double) * total) != hipSuccess)
+    return set_err(c, RAPT_E_OOM, "grid allocation of %zu bytes failed", sizeof(double) * total);
+  if (hipMemcpy(mem, h.data(), sizeof(double) * total, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(mem);
+    return set_err(c, RAPT_E_HIP, "grid upload failed");
+  }
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(c->grid_mem);
+  c->grid_mem = mem;
+  GridC& G = c->grid;
+  G = GridC{};
+  for (int k = 0; k < 4; ++k) {
+    G.ax[k] = mem + off[k];
+    G.n[k] = n[k];
+    if (k == 3 && nt == 1) continue;
+    // uniform spacing: every coordinate within 1e-12 of the spacing of a0 + i h
+    const double h0 = (ax[k][n[k] - 1] - ax[k][0]) / (n[k] - 1);
+    bool uni = true;
+    for (int32_t i = 0; i < n[k] && uni; ++i)
+      uni = fabs(ax[k][i] - (ax[k][0] + i * h0)) <= 1e-12 * fabs(h0) * n[k];
+    if (uni) {
+      G.uniform |= 1 << k;
+      G.a0[k] = ax[k][0];
+      G.ida[k] = 1.0 / h0;
+    }
+  }
+  G.B = mem + offB;
+  G.E = E ? mem + offE : nullptr;
+  c->grid_set = true;
+  if (c->field_set && c->field.model == RAPT_FIELD_GRID) c->field.grid = G;
   return RAPT_OK;
 }
 
@@ -215,6 +285,7 @@ int rapt_field_eval(rapt_ctx* c, int what, int64_t n, const double* tpos, double
       case 6: launch_field_eval<6>(c, what, n, dt, dout, w); break;
       case 7: launch_field_eval<7>(c, what, n, dt, dout, w); break;
       case 8: launch_field_eval<8>(c, what, n, dt, dout, w); break;
+      case 9: launch_field_eval<9>(c, what, n, dt, dout, w); break;
     }
     if (hipGetLastError() != hipSuccess) {
       rc = set_err(c, RAPT_E_HIP, "field kernel launch failed");
@@ -397,6 +468,8 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
     return set_err(c, RAPT_E_ARG, "work_order must be 0 or 1");
   if (d->precision != RAPT_PREC_FP64 && d->precision != RAPT_PREC_FP32_FIELD)
     return set_err(c, RAPT_E_ARG, "precision must be RAPT_PREC_FP64 or RAPT_PREC_FP32_FIELD");
+  if (d->precision == RAPT_PREC_FP32_FIELD && c->field.model == RAPT_FIELD_GRID)
+    return set_err(c, RAPT_E_ARG, "gridded fields are traced in FP64 only");
   if (d->mode == RAPT_MODE_ADAPTIVE && !(d->gctimestep > 0.0))
     return set_err(c, RAPT_E_ARG, "ADAPTIVE needs params['GCtimestep'] > 0");
   if (d->mode == RAPT_MODE_PARTICLE && !(d->cyclotronresolution > 0.0))

@@ -9,9 +9,12 @@ HIP library, whose kernels evaluate the same formulas on the device.  A user
 subclass that only overrides ``B``/``E`` in Python has no descriptor and
 cannot be traced on the GPU: ``descriptor()`` raises NotImplementedError.
 
-Reference: rapt/fields.py:5-511.  ``Grid`` (rapt/fields.py:513-814) is not
-provided.
+Reference: rapt/fields.py:5-814.  ``Grid`` (rapt/fields.py:513-814) keeps
+every parsed snapshot resident in device memory and is interpolated on the
+device (trilinear in space, linear in time), see ``Grid``.
 """
+import itertools
+
 import numpy as np
 
 from .constants import B0, Re
@@ -26,6 +29,7 @@ FIELD_VAR_EARTH_DIPOLE = 5
 FIELD_PARABOLIC = 6
 FIELD_CURRENT_SHEET = 7
 FIELD_DIPOLE_UNIFORM = 8
+FIELD_GRID = 9
 
 
 class _Field:
@@ -296,11 +300,108 @@ class DipolePlusUniform(EarthDipole):
         return EarthDipole.B(self, tpos) + self._Bu
 
 
+_grid_ids = itertools.count(1)
+
+
+class Grid(_Field):
+    """Fields sampled on a Cartesian grid (rapt/fields.py:513-814).
+
+    Subclass it and override ``parsefile(filename)`` to return a dict with
+    "time" (float), "x", "y", "z" (1-D, strictly increasing; uniform spacing
+    not required) and "Bx", "By", "Bz", "Ex", "Ey", "Ez" (3-D arrays indexed
+    [x, y, z]), all SI, as the reference's contract states.
+
+    Differences from the reference, by design:
+
+    - every file is parsed at construction and all snapshots stay resident
+      on the GPU (288 GB of HBM holds long MHD runs); the reference keeps a
+      sliding window of three snapshots and forgets earlier times.  Within
+      any window the interpolation is identical: linear in t between the
+      bracketing snapshots, trilinear in space (scipy's
+      RegularGridInterpolator, method "linear");
+    - a single file gives a time-independent 3-D grid (the reference's
+      constructor sets ``time_indep`` instead of ``_time_indep`` and then
+      calls its 3-D interpolator with four arguments);
+    - outside the grid the host methods raise ValueError as the reference
+      does; on the device B and E are NaN there and a traced particle stops
+      with a non-finite or step-size status.
+
+    ``B``/``E`` may not be overridden for tracing (the device interpolates the
+    parsed data only).  ``static`` stays True unless the subclass sets it, as
+    in the reference.
+    """
+    _model = FIELD_GRID
+
+    def __init__(self, filelist):
+        assert len(filelist) > 0
+        _Field.__init__(self)
+        self.gradientstepsize = 1e-3 * Re
+        self.files = list(filelist)
+        gs = [self.parsefile(f) for f in self.files]
+        g0 = gs[0]
+        self._x, self._y, self._z = (np.asarray(g0[k], dtype=float) for k in ("x", "y", "z"))
+        shape = (len(self._x), len(self._y), len(self._z))
+        for g in gs:
+            for k in ("Bx", "By", "Bz", "Ex", "Ey", "Ez"):
+                if np.shape(g[k]) != shape:
+                    raise ValueError("%s has shape %s, grid is %s" % (k, np.shape(g[k]), shape))
+        self._t = np.array([float(g["time"]) for g in gs])
+        self._B = np.ascontiguousarray(
+            np.stack([np.stack([g["Bx"], g["By"], g["Bz"]], axis=-1) for g in gs]), dtype=float)
+        self._E = np.ascontiguousarray(
+            np.stack([np.stack([g["Ex"], g["Ey"], g["Ez"]], axis=-1) for g in gs]), dtype=float)
+        self._time_indep = len(gs) == 1
+        self._grid_id = next(_grid_ids)
+        self._interp = None
+
+    def parsefile(self, filename):
+        """Parse one snapshot (override when subclassing)."""
+        return dict()
+
+    def grid_arrays(self):
+        """(x, y, z, t, B [nt][nx][ny][nz][3], E likewise) as uploaded to the device."""
+        return self._x, self._y, self._z, self._t, self._B, self._E
+
+    def _interpolators(self):
+        if self._interp is None:
+            from scipy.interpolate import RegularGridInterpolator as RGI
+            ax = (self._x, self._y, self._z)
+            if self._time_indep:
+                self._interp = (RGI(ax, self._B[0]), RGI(ax, self._E[0]))
+            else:
+                ax = (self._t,) + ax
+                self._interp = (RGI(ax, self._B), RGI(ax, self._E))
+        return self._interp
+
+    def Bgrid(self, tpos):
+        tpos = np.asarray(tpos, dtype=float)
+        q = tpos[1:] if self._time_indep else tpos
+        return self._interpolators()[0](q)[0]
+
+    def Egrid(self, tpos):
+        tpos = np.asarray(tpos, dtype=float)
+        q = tpos[1:] if self._time_indep else tpos
+        return self._interpolators()[1](q)[0]
+
+    def B(self, tpos):
+        return self.Bgrid(tpos)
+
+    def E(self, tpos):
+        return self.Egrid(tpos)
+
+    def descriptor(self):
+        d = _Field.descriptor(self)
+        d["grid"] = self
+        return d
+
+
 _MODEL_B = {FIELD_EARTH_DIPOLE: EarthDipole.B, FIELD_DOUBLE_DIPOLE: DoubleDipole.B,
             FIELD_UNIFORM_BZ: UniformBz.B, FIELD_UNIFORM_CROSSED_EB: UniformBz.B,
             FIELD_VAR_EARTH_DIPOLE: VarEarthDipole.B, FIELD_PARABOLIC: Parabolic.B,
-            FIELD_CURRENT_SHEET: CurrentSheet.B, FIELD_DIPOLE_UNIFORM: DipolePlusUniform.B}
+            FIELD_CURRENT_SHEET: CurrentSheet.B, FIELD_DIPOLE_UNIFORM: DipolePlusUniform.B,
+            FIELD_GRID: Grid.B}
 _MODEL_E = {FIELD_EARTH_DIPOLE: _Field.E, FIELD_DOUBLE_DIPOLE: _Field.E,
             FIELD_UNIFORM_BZ: _Field.E, FIELD_UNIFORM_CROSSED_EB: UniformCrossedEB.E,
             FIELD_VAR_EARTH_DIPOLE: _Field.E, FIELD_PARABOLIC: _Field.E,
-            FIELD_CURRENT_SHEET: CurrentSheet.E, FIELD_DIPOLE_UNIFORM: _Field.E}
+            FIELD_CURRENT_SHEET: CurrentSheet.E, FIELD_DIPOLE_UNIFORM: _Field.E,
+            FIELD_GRID: Grid.E}
