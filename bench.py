@@ -89,7 +89,7 @@ def cpu_baseline(w, sample, threads):
     el = time.perf_counter() - t0
     return cnt.naccpt, el
 
-def timed_run(field, mode, S, flags, cap, desc, steps, warmup, width, rows=False):
+def timed_run(field, mode, S, flags, cap, desc, steps, warmup, width, rows=False, cost=None):
     """Upload once, then warmup + timed runs of the same batch (state reset on
     the device between runs).  Returns (totals, seconds, last download)."""
     import ctypes
@@ -98,6 +98,8 @@ def timed_run(field, mode, S, flags, cap, desc, steps, warmup, width, rows=False
     ctx.set_field(field.descriptor())
     b = N.Batch(ctx, S.shape[1])
     b.upload(S, flags)
+    if cost is not None:
+        b.set_cost(cost)
     b.set_rows(width, cap)
     for i in range(warmup):
         b.reset()
@@ -164,6 +166,88 @@ def run_particle_dipole(args):
     res["accepted_steps_per_step"] = tot["naccpt"] / args.steps
     res["kernel_ms_avg"] = tot["kernel_ms"] / args.steps
     res["particles_done"] = int(np.sum(out["status"] == 1))
+    print(json.dumps(res), flush=True)
+
+
+def run_grid(args):
+    """Particle.advance in a gridded field: EarthDipole sampled on a uniform
+    257 x 257 x 193 grid over |x|, |y| <= 8 Re, |z| <= 6 Re (B only, 408 MB
+    in HBM), the config-1 proton population at n = 65,536, advance(10),
+    DOP853; the same batch in the analytic dipole for comparison."""
+    import rapt_amd as rapt
+    from rapt_amd import _native as N, batch as BT, workloads as W
+    n = args.n or 65536
+    Re = rapt.Re
+    ax = (np.linspace(-8, 8, 257) * Re, np.linspace(-8, 8, 257) * Re,
+          np.linspace(-6, 6, 193) * Re)
+
+    class SampledDipole(rapt.fields.Grid):
+        def parsefile(self, filename):
+            x, y, z = ax
+            X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+            r2 = X * X + Y * Y + Z * Z
+            with np.errstate(divide="ignore", invalid="ignore"):
+                s = rapt.fields.EarthDipole()._coeff / r2 ** 2.5
+            s[~np.isfinite(s)] = 0.0
+            zero = np.zeros_like(X)
+            return dict(time=0.0, x=x, y=y, z=z, Bx=s * X * Z, By=s * Y * Z,
+                        Bz=s * (Z * Z - r2 / 3), Ex=zero, Ey=zero, Ez=zero)
+
+    t0 = time.perf_counter()
+    grid = SampledDipole(["dipole"])
+    log("grid sampled in %.1f s" % (time.perf_counter() - t0))
+    ana = rapt.fields.EarthDipole()
+    wl = W.dipole_protons_1mev(n=n)
+    g = 1 / np.sqrt(1 - np.sum(wl["vel"] ** 2, axis=1) / rapt.c ** 2)
+    last7 = np.zeros((n, 7))
+    last7[:, 1:4] = wl["pos"]
+    last7[:, 4:7] = (rapt.m_pr * g)[:, None] * wl["vel"]
+    S = BT.particle_state(last7, rapt.m_pr, rapt.e)
+    dt = BT.particle_dt(ana, last7, rapt.m_pr, rapt.e, 20)
+    S[N.S_DT] = dt
+    cap = BT._rows_needed(10.0, dt)
+    desc = N.make_desc(N.MODE_PARTICLE, dict(rapt.params), 10.0, lanes=args.lanes)
+    # work order for the gathers: particles whose start cells are close in
+    # Z order share wavefronts, so a wave's corner reads share cache lines
+    cells = [np.clip(((last7[:, 1 + k] - a[0]) / (a[1] - a[0])).astype(np.int64), 0, 1023)
+             for k, a in enumerate(ax)]
+    code = np.zeros(n, np.int64)
+    for bit in range(10):
+        for k in range(3):
+            code |= ((cells[k] >> bit) & 1) << (3 * bit + k)
+    cost = None if args.order != "cost" else -code.astype(np.float64)
+    runs = {}
+    for name, f in (("analytic", ana), ("grid", grid)):
+        tot, el, out = timed_run(f, N.MODE_PARTICLE, S, np.zeros(n, np.int32), cap, desc,
+                                 args.steps, args.warmup, 7, cost=cost if name == "grid" else None)
+        runs[name] = (tot, el, out)
+    tot, el, out = runs["grid"]
+    value = tot["naccpt"] / el
+    evals = tot["nfcn"] / args.steps
+    gather = evals * 8 * 24.0          # 8 cell corners x (Bx, By, Bz) per evaluation
+    ksec = tot["kernel_ms"] / args.steps / 1e3
+    fa, fg = runs["analytic"][2]["state"], out["state"]
+    pos_a, pos_g = fa[1:4], fg[1:4]
+    rel = np.linalg.norm(pos_g - pos_a, axis=0) / np.linalg.norm(pos_a, axis=0)
+    res = base_line("FP64 particle-steps/s in a gridded field (1 GPU)", value,
+                    "particle-steps/s", args, el, "f64",
+                    {"workload": "EarthDipole sampled on a uniform 257 x 257 x 193 grid "
+                                 "(|x|,|y| <= 8 Re, |z| <= 6 Re, 408 MB of B in HBM); %d protons "
+                                 "1 MeV, r U[4,7] Re, pitch U[10,90] deg; DOP853 default "
+                                 "tolerances, advance(10), dt=T_c/20" % n, "particles": n})
+    res.update({
+        "kernel_ms_avg": tot["kernel_ms"] / args.steps,
+        "accepted_steps_per_step": tot["naccpt"] / args.steps,
+        "field_evaluations_per_step": evals,
+        "analytic_dipole_steps_per_s": runs["analytic"][0]["naccpt"] / runs["analytic"][1],
+        "analytic_dipole_kernel_ms_avg": runs["analytic"][0]["kernel_ms"] / args.steps,
+        "final_position_rel_diff_grid_vs_analytic": _pct(rel),
+        "particles_done": int(np.sum(out["status"] == 1)),
+        "roofline": {"bound": "hbm", "achieved": gather / ksec / 1e9, "peak": PEAK_HBM / 1e9,
+                     "unit": "GB/s", "frac": gather / ksec / PEAK_HBM, "traffic": None,
+                     "algorithmic_bytes_per_launch": gather,
+                     "note": "gather bytes if every corner read came from HBM; L2/MALL "
+                             "reuse between nearby evaluations lowers the real traffic"}})
     print(json.dumps(res), flush=True)
 
 
@@ -438,7 +522,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="gc_doubledipole",
                     choices=("gc_doubledipole", "particle_dipole", "adaptive_sheet",
-                             "mixed_precision", "surrogate"))
+                             "mixed_precision", "surrogate", "grid"))
     ap.add_argument("--epochs", type=int, default=60, help="surrogate training epochs")
     ap.add_argument("--dump", default="", help="surrogate: write held-out results to this .npz")
     ap.add_argument("--steps", type=int, default=3)
@@ -480,6 +564,7 @@ def main():
 
     if args.workload != "gc_doubledipole":
         {"particle_dipole": run_particle_dipole, "adaptive_sheet": run_adaptive_sheet,
+         "grid": run_grid,
          "mixed_precision": run_mixed_precision, "surrogate": run_surrogate}[args.workload](args)
         return
     args.n = args.n or 65536

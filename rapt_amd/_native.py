@@ -177,8 +177,10 @@ class Context:
             x, y, z, t, B, E = (np.ascontiguousarray(a, dtype=np.float64)
                                 for a in grid.grid_arrays())
             nt = B.shape[0]
+            # an all-zero E is not uploaded: the device then skips E (E = 0)
             self.check(lib().rapt_grid_set(self.h, len(x), len(y), len(z), nt, _dp(x), _dp(y),
-                                           _dp(z), _dp(t), _dp(B), _dp(E)), "rapt_grid_set")
+                                           _dp(z), _dp(t), _dp(B),
+                                           _dp(E) if np.any(E) else None), "rapt_grid_set")
             self._grid_id = grid._grid_id
         f = FieldDesc()
         f.model = desc["model"]

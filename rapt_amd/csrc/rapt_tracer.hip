@@ -239,6 +239,8 @@ int rapt_grid_set(rapt_ctx* c, int32_t nx, int32_t ny, int32_t nz, int32_t nt, c
     G.ax[k] = mem + off[k];
     G.n[k] = n[k];
     if (k == 3 && nt == 1) continue;
+    G.lo[k] = ax[k][0];
+    G.hi[k] = ax[k][n[k] - 1];
     // uniform spacing: every coordinate within 1e-12 of the spacing of a0 + i h
     const double h0 = (ax[k][n[k] - 1] - ax[k][0]) / (n[k] - 1);
     bool uni = true;
