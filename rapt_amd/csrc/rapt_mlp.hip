@@ -507,9 +507,10 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
         for (int r = 0; r < 4; ++r) {
           const float4 c = sw.l3[16 * t + 4 * h + r];
           const float v = silu(acc[r] + c.x);
-          o0 += c.y * v;
-          o1 += c.z * v;
-          o2 += c.w * v;
+          // fused multiply-adds (the library compiles without contraction)
+          o0 = __builtin_fmaf(c.y, v, o0);
+          o1 = __builtin_fmaf(c.z, v, o1);
+          o2 = __builtin_fmaf(c.w, v, o2);
         }
       }
       o0 += __shfl_xor(o0, 16);
