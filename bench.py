@@ -529,7 +529,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--lpp", type=int, default=4, choices=(1, 2, 4),
+    ap.add_argument("--mixed-quads", type=int, default=0,
+                    help="with --lpp 3: particles traced on lane quads (0: n/16)")
+    ap.add_argument("--lpp", type=int, default=4, choices=(1, 2, 3, 4),
                     help="lanes per guiding centre (4: a lane quad, 2: a lane pair per "
                          "particle); with --no-tune")
     ap.add_argument("--block", type=int, default=0,
@@ -578,7 +580,8 @@ def main():
     flags = np.zeros(n, np.int32)
     cap = BT._rows_needed(w["delta"], w["dt"])
     desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes, block=args.block,
-                       lanes_per_particle=args.lpp, work_order=1 if args.order == "input" else 0)
+                       lanes_per_particle=args.lpp, work_order=1 if args.order == "input" else 0,
+                       mixed_quads=args.mixed_quads)
     cost = None
     if args.order == "cost":
         from rapt_amd import autotune as AT

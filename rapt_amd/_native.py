@@ -54,7 +54,7 @@ class TraceDesc(C.Structure):
                 ("cyclotronresolution", C.c_double), ("gctimestep", C.c_double),
                 ("epss", C.c_double), ("epst", C.c_double), ("delta", C.c_double),
                 ("lanes", C.c_int64), ("lanes_per_particle", C.c_int32),
-                ("work_order", C.c_int32)]
+                ("work_order", C.c_int32), ("mixed_quads", C.c_int64)]
 
 
 class Counters(C.Structure):
@@ -428,4 +428,5 @@ def make_desc(mode, params, delta, eom="TaoChanBrizardEOM", stepper=None, check=
     d.lanes = int(lanes)
     d.lanes_per_particle = int(kw.get("lanes_per_particle", 0))
     d.work_order = int(kw.get("work_order", 0))
+    d.mixed_quads = int(kw.get("mixed_quads", 0))
     return d

@@ -33,6 +33,12 @@ def candidates(mode, stepper, precision, n):
             out.append(dict(lanes_per_particle=4, block=block, lanes=0))
             out.append(dict(lanes_per_particle=2, block=block, lanes=0))
         out.append(dict(lanes_per_particle=1, block=block, lanes=0))
+    # mixed: the most expensive particles (the front of the cost order) on
+    # quads, the rest on pairs
+    if grouped_ok:
+        for frac in (16, 64):
+            out.append(dict(lanes_per_particle=3, block=256, lanes=0,
+                            mixed_quads=max(1, n // frac)))
     # fewer resident lanes than particles: refill from the queue sooner
     if n >= 8 * 65536:
         out.append(dict(lanes_per_particle=1, block=256, lanes=4 * 65536))

@@ -270,9 +270,12 @@ __device__ __forceinline__ CV<4 / L> rhs_gcn(const FieldC& F, int eom, double ch
 #define RAPT_GCN_WAVES 2
 #endif
 
+// The tracer loop of one lane group.  The work queue is split in two: queue
+// 0 holds work items [0, A.split) and queue 1 [A.split, n) (counters
+// A.work[0] and A.work[7]).  A group takes items from queue qown first and
+// from the other queue once its own is drained.
 template <int M, int L>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAPT_GCN_WAVES)))
-void k_trace_gcn(KArgs A) {
+__device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   constexpr int NC = 4 / L;
   using Y4 = CV<NC>;
   const FieldC& F = A.F;
@@ -290,6 +293,7 @@ void k_trace_gcn(KArgs A) {
   double xend = 0, hmax = 0, facold = 1e-4;
   int nstep = 0, nacc_call = 0;
   bool last = false, reject = false, k1ok = false;
+  bool drained[2] = {false, false};
   int64_t nrow = 0, rowcap = 0, rowbase = 0;
   int64_t c_acc = 0, c_rej = 0, c_stp = 0, c_fcn = 0, p_acc = 0;
   const int W = 5;
@@ -334,9 +338,20 @@ void k_trace_gcn(KArgs A) {
 
   for (;;) {
     if (!active && !exhausted) {
-      int64_t got = 0;
+      int64_t got = n;
       if (q == 0) {
-        got = (int64_t)atomicAdd(&A.work[0], 1ULL);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const int qq = k == 0 ? qown : 1 - qown;
+          if (got < n || drained[qq]) continue;
+          const int64_t lo = qq == 0 ? 0 : A.split, hi = qq == 0 ? A.split : n;
+          const int64_t t = (int64_t)atomicAdd(&A.work[qq == 0 ? 0 : 7], 1ULL);
+          if (t < hi - lo) {
+            got = lo + t;
+          } else {
+            drained[qq] = true;
+          }
+        }
         if (got < n && A.order) got = A.order[got];
       }
       pi = gbll0<L>(got);
@@ -554,6 +569,28 @@ void k_trace_gcn(KArgs A) {
   }
 }
 
+template <int M, int L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAPT_GCN_WAVES)))
+void k_trace_gcn(KArgs A) {
+  gcn_body<M, L>(A, 0);
+}
+
+// Mixed layout: blocks [0, quad_blocks) trace with lane quads and take queue
+// 0 first (the most expensive particles in cost order: their step chains
+// are the launch's critical path, and a quad's step is the shortest), the
+// other blocks with lane pairs (fewer instructions per particle-step) and
+// take queue 1 first.  Results do not depend on which layout traced a
+// particle.
+template <int M>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAPT_GCN_WAVES)))
+void k_trace_gcmix(KArgs A, int32_t quad_blocks) {
+  if ((int32_t)blockIdx.x < quad_blocks) {
+    gcn_body<M, 4>(A, 0);
+  } else {
+    gcn_body<M, 2>(A, 1);
+  }
+}
+
 }  // namespace rapt
 
 using namespace rapt;
@@ -581,10 +618,32 @@ static void launch_gcn_l(hipStream_t st, int model, const KArgs& A, int block, i
   }
 }
 
-// groups resident particle groups of L = 2 or 4 lanes each
+template <int M>
+static void launch_mix_m(hipStream_t st, const KArgs& A, int block, int64_t quads,
+                         int64_t pairs) {
+  const int64_t qb = (4 * quads + block - 1) / block, pb = (2 * pairs + block - 1) / block;
+  hipLaunchKernelGGL((k_trace_gcmix<M>), dim3((unsigned)(qb + pb)), dim3(block), 0, st, A,
+                     (int32_t)qb);
+}
+
+// groups resident particle groups of L = 2 or 4 lanes each; L = 3 is the
+// mixed layout with `groups` quads and `pairs` pairs
 void rapt_launch_gcn(hipStream_t st, int model, int lanes_per_particle, const KArgs& A,
-                     int block, int64_t groups) {
-  if (lanes_per_particle == 2) {
+                     int block, int64_t groups, int64_t pairs) {
+  if (lanes_per_particle == 3) {
+    switch (model) {
+      case 0: launch_mix_m<0>(st, A, block, groups, pairs); break;
+      case 1: launch_mix_m<1>(st, A, block, groups, pairs); break;
+      case 2: launch_mix_m<2>(st, A, block, groups, pairs); break;
+      case 3: launch_mix_m<3>(st, A, block, groups, pairs); break;
+      case 4: launch_mix_m<4>(st, A, block, groups, pairs); break;
+      case 5: launch_mix_m<5>(st, A, block, groups, pairs); break;
+      case 6: launch_mix_m<6>(st, A, block, groups, pairs); break;
+      case 7: launch_mix_m<7>(st, A, block, groups, pairs); break;
+      case 8: launch_mix_m<8>(st, A, block, groups, pairs); break;
+      case 9: launch_mix_m<9>(st, A, block, groups, pairs); break;
+    }
+  } else if (lanes_per_particle == 2) {
     launch_gcn_l<2>(st, model, A, block, groups);
   } else {
     launch_gcn_l<4>(st, model, A, block, groups);

@@ -44,6 +44,7 @@ struct KArgs {
   double* rows;
   unsigned long long* work;  // [0] next work item, [1..6] counters
   const int64_t* order;      // work item -> particle index (nullptr: identity)
+  int64_t split;             // grouped GC kernels: items [0, split) form queue 0
 };
 
 }  // namespace rapt

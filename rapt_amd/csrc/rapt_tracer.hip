@@ -114,7 +114,7 @@ static CtlParams ctl(double safety, double dfactor, double ifactor, double beta,
 }
 
 void rapt_launch_gcn(hipStream_t st, int model, int lanes_per_particle, const KArgs& A,
-                     int block, int64_t groups);
+                     int block, int64_t groups, int64_t pairs);
 
 void rapt_launch_trace_f32(hipStream_t st, int model, int kind, const KArgs& A, int block,
                            int64_t lanes);
@@ -463,9 +463,9 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
     return set_err(c, RAPT_E_ARG, "PARTICLE mode steps with DOP853 or RKF45");
   if (d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOP853)
     return set_err(c, RAPT_E_ARG, "GC mode steps with DOPRI5 or RKF45");
-  if (d->lanes_per_particle != 0 && d->lanes_per_particle != 1 && d->lanes_per_particle != 2 &&
-      d->lanes_per_particle != 4)
-    return set_err(c, RAPT_E_ARG, "lanes_per_particle must be 0, 1, 2 or 4");
+  if (d->lanes_per_particle < 0 || d->lanes_per_particle > 4)
+    return set_err(c, RAPT_E_ARG, "lanes_per_particle must be 0-4");
+  if (d->mixed_quads < 0) return set_err(c, RAPT_E_ARG, "mixed_quads must be >= 0");
   if (d->work_order != 0 && d->work_order != 1)
     return set_err(c, RAPT_E_ARG, "work_order must be 0 or 1");
   if (d->precision != RAPT_PREC_FP64 && d->precision != RAPT_PREC_FP32_FIELD)
@@ -533,6 +533,11 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   A.rows = b->d_rows;
   A.work = b->d_work;
   A.order = d->work_order == 0 ? b->d_order : nullptr;
+  A.split = b->n;
+  if (d->lanes_per_particle == 3) {
+    const int64_t k = d->mixed_quads > 0 ? d->mixed_quads : b->n / 16;
+    A.split = k < b->n ? (k > 0 ? k : 1) : b->n;
+  }
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
   if (lanes > b->n) lanes = b->n;
   if (lanes < 1) lanes = 1;
@@ -549,10 +554,15 @@ static void launch_tracer(rapt_batch* b, const rapt_trace_desc* d, const KArgs& 
                           int64_t lanes, bool grouped) {
   rapt_ctx* c = b->ctx;
   if (grouped) {
-    const int L = d->lanes_per_particle == 2 ? 2 : 4;
-    int64_t groups = d->lanes > 0 ? (d->lanes + L - 1) / L : b->n;
-    if (groups > b->n) groups = b->n;
-    rapt_launch_gcn(c->stream, c->field.model, L, A, block, groups);
+    if (d->lanes_per_particle == 3) {
+      // every particle resident: A.split quads, the rest pairs
+      rapt_launch_gcn(c->stream, c->field.model, 3, A, block, A.split, b->n - A.split);
+    } else {
+      const int L = d->lanes_per_particle == 2 ? 2 : 4;
+      int64_t groups = d->lanes > 0 ? (d->lanes + L - 1) / L : b->n;
+      if (groups > b->n) groups = b->n;
+      rapt_launch_gcn(c->stream, c->field.model, L, A, block, groups, 0);
+    }
   } else if (d->precision == RAPT_PREC_FP32_FIELD) {
     const int kind = d->mode == RAPT_MODE_PARTICLE ? KIND_PARTICLE
                                                    : (d->mode == RAPT_MODE_GC ? KIND_GC
