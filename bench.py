@@ -591,11 +591,12 @@ def main():
     tuned = None
     if not args.no_tune:
         from rapt_amd import autotune as AT
-        shape, table = AT.tune(w["field"], S, flags, cap, desc, trial_fraction=0.25, log=log,
+        shape, table = AT.tune(w["field"], S, flags, cap, desc, trial_fraction=0.5, log=log,
                                cost=cost)
         AT.apply(desc, shape)
         tuned = {"chosen": shape, "trials": [{"shape": t, "steps_per_s": r} for t, r in table],
-                 "trial_fraction_of_delta": 0.25, "trial_work_order": args.order}
+                 "trial_fraction_of_delta": 0.5, "trial_work_order": args.order,
+                 "trial_repeats": 2}
         log("autotune chose %s" % shape)
     b = N.Batch(ctx, n)
     b.upload(S, flags)

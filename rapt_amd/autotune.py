@@ -7,7 +7,8 @@ time -- and at 1,048,576 the one-lane kernel wins (enough particles to fill
 the chip, and it spends the fewest issue slots per step).  The tuner times short trial launches of the real batch -- the same
 particles, the same work order, a fraction of the advance -- for each
 candidate (block size, lanes per particle, resident lanes, i.e. the work
-refill threshold) and keeps the one with the highest accepted-steps rate.
+refill threshold) and keeps the one with the highest accepted-steps rate
+(best of two launches each).
 Results are cached per (mode, stepper, field model, precision, equations,
 batch size rounded to a power of two).
 
@@ -51,7 +52,7 @@ def key(desc, model, n, ordered=False):
 
 
 def tune(field, S, flags, row_cap, desc, trial_fraction=0.1, device=0, force=False, log=None,
-         cost=None):
+         cost=None, repeats=2):
     """Choose a launch shape for this batch.  Returns (best dict, table of
     (shape, accepted steps/s)).  desc is not modified.  cost: the work order
     the run will use (Batch.set_cost), applied to the trials too, so that
@@ -76,12 +77,17 @@ def tune(field, S, flags, row_cap, desc, trial_fraction=0.1, device=0, force=Fal
         for cand in candidates(desc.mode, desc.stepper, desc.precision, n):
             for f, v in cand.items():
                 setattr(trial, f, v)
-            b.reset()
-            c = b.run(trial)
-            rate = c.naccpt / max(c.kernel_ms, 1e-6) * 1e3
+            # best of `repeats` launches: the first launch of a kernel also
+            # loads its code, and single trials vary by ~10% between runs
+            best_ms = float("inf")
+            for _ in range(repeats):
+                b.reset()
+                c = b.run(trial)
+                best_ms = min(best_ms, c.kernel_ms)
+            rate = c.naccpt / max(best_ms, 1e-6) * 1e3
             table.append((dict(cand), rate))
             if log:
-                log("autotune %s: %.3g accepted steps/s (%.1f ms)" % (cand, rate, c.kernel_ms))
+                log("autotune %s: %.3g accepted steps/s (%.1f ms)" % (cand, rate, best_ms))
     finally:
         b.close()
     best = max(table, key=lambda t: t[1])[0]
