@@ -39,10 +39,11 @@ FLOP_RHS_TCB_REFERENCE = 795         # same RHS as the reference writes it (13 f
 PEAK_FP64 = 78.6e12                  # MI355X FP64 vector (and matrix) peak, FLOP/s
 PEAK_HBM = 8.0e12                    # bytes/s
 ROW_BYTES_GC = 40                    # one (t, x, y, z, ppar) output row
-# HBM bytes per k_trace_gcn launch at the default batch, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE (KiB) scaled by a 256 MiB copy probe (read x2.00,
-# write x1.00); profiles/k_trace_gcn_doubledipole_65536.md has the commands
-PROFILED_TRAFFIC = {65536: (57870.0 * 2.0 + 2831037.25 * 1.0) * 1024}
+# HBM bytes per launch of the default batch's kernel (k_trace_gcmix, the
+# autotuner's choice), from rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB) scaled by
+# a 256 MiB copy probe (read x2.00, write x1.00);
+# profiles/k_trace_gcmix_doubledipole_65536.md has the commands
+PROFILED_TRAFFIC = {65536: (42844.0 * 2.0 + 3068078.34375 * 1.0) * 1024}
 
 
 def log(*a):
