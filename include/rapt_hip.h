@@ -159,8 +159,12 @@ int rapt_field_set(rapt_ctx *ctx, const rapt_field_desc *field);
  * snapshots resident (the reference keeps a window of three).  Evaluation is
  * trilinear in space and linear in time between the bracketing snapshots;
  * outside the grid B and E are NaN (the reference raises ValueError), which
- * stops a traced particle with a non-finite or step-size status.  Then
- * rapt_field_set with model RAPT_FIELD_GRID (params unused) selects it. */
+ * stops a traced particle with a non-finite or step-size status.  The
+ * device copy is cell-major (the 8 corners of each cell contiguous, 8x the
+ * point data) when that takes at most 64 GiB and half the free device
+ * memory, else point-major; environment RAPT_GRID_POINTS=1 forces the point
+ * layout.  Both give identical values.  Then rapt_field_set with model
+ * RAPT_FIELD_GRID (params unused) selects it. */
 int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt, const double *x,
                   const double *y, const double *z, const double *t, const double *B,
                   const double *E);

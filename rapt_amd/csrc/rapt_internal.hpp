@@ -56,7 +56,8 @@ struct rapt_ctx {
   rapt::FieldC field{};
   bool field_set = false;
   rapt::GridC grid{};         // uploaded by rapt_grid_set (device memory below)
-  double* grid_mem = nullptr;
+  double* grid_mem = nullptr;    // axes and point-layout values
+  double* grid_cells = nullptr;  // cell-layout values (nullptr: point layout)
   bool grid_set = false;
   std::string err;
 };

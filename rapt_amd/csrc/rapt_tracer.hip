@@ -32,6 +32,32 @@ __global__ void k_epoch_prep(unsigned long long* work, int32_t* left) {
   if (threadIdx.x < 8) work[threadIdx.x] = 0ULL;
   if (threadIdx.x == 0) *left = 0;
 }
+
+// Gridded fields: expand the point layout [nt][nx][ny][nz][4] into the cell
+// layout [nt][nx-1][ny-1][nz-1][32] (the 8 corners of each cell, corner
+// (dx, dy, dz) at 3 (4 dx + 2 dy + dz), the last 8 doubles zero).  One thread
+// per (cell, corner).
+__global__ void k_grid_cells(const double* P, double* C, int32_t nx, int32_t ny, int32_t nz,
+                             int64_t ncell) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ncell * 8) return;
+  const int corner = (int)(idx & 7);
+  int64_t cell = idx >> 3;
+  const int dz = corner & 1, dy = (corner >> 1) & 1, dx = corner >> 2;
+  const int64_t k = cell % (nz - 1);
+  int64_t r = cell / (nz - 1);
+  const int64_t j = r % (ny - 1);
+  r /= (ny - 1);
+  const int64_t i = r % (nx - 1);
+  const int64_t it = r / (nx - 1);
+  const double* p = P + (((it * nx + i + dx) * ny + j + dy) * nz + k + dz) * 4;
+  double* c = C + cell * 32 + 3 * corner;
+  c[0] = p[0];
+  c[1] = p[1];
+  c[2] = p[2];
+  if (corner == 7)
+    for (int m = 0; m < 8; ++m) C[cell * 32 + 24 + m] = 0.0;
+}
 __global__ void k_epoch_left(int64_t n, const int32_t* status, int32_t* left) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -168,6 +194,7 @@ int rapt_ctx_destroy(rapt_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   (void)hipFree(c->grid_mem);
+  (void)hipFree(c->grid_cells);
   delete c;
   return RAPT_OK;
 }
@@ -231,8 +258,38 @@ int rapt_grid_set(rapt_ctx* c, int32_t nx, int32_t ny, int32_t nz, int32_t nt, c
     return set_err(c, RAPT_E_HIP, "grid upload failed");
   }
   HIP_OK(c, hipStreamSynchronize(c->stream));
+  // Cell layout when it fits (8x the point data; at most 64 GiB and half the
+  // free device memory): one aligned 192-byte run per evaluation instead of
+  // four 64-byte runs.  Otherwise the point layout stays.
+  const size_t ncell = (size_t)(nx - 1) * (ny - 1) * (nz - 1) * nt;
+  const size_t cell_bytes = sizeof(double) * 32 * ncell * (E ? 2 : 1);
+  size_t free_b = 0, total_b = 0;
+  (void)hipMemGetInfo(&free_b, &total_b);
+  double* cells = nullptr;
+  const char* pts = getenv("RAPT_GRID_POINTS");  // "1": keep the point layout
+  const bool want_cells = !(pts && pts[0] == '1');
+  if (want_cells && cell_bytes <= ((size_t)64 << 30) && cell_bytes <= free_b / 2 &&
+      hipMalloc(&cells, cell_bytes) == hipSuccess) {
+    const int64_t nthr = (int64_t)ncell * 8;
+    const unsigned nblk = (unsigned)((nthr + 255) / 256);
+    hipLaunchKernelGGL(k_grid_cells, dim3(nblk), dim3(256), 0, c->stream, mem + offB, cells,
+                       nx, ny, nz, (int64_t)ncell);
+    if (E)
+      hipLaunchKernelGGL(k_grid_cells, dim3(nblk), dim3(256), 0, c->stream, mem + offE,
+                         cells + 32 * ncell, nx, ny, nz, (int64_t)ncell);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+      (void)hipFree(cells);
+      (void)hipFree(mem);
+      return set_err(c, RAPT_E_HIP, "grid cell layout failed");
+    }
+  } else {
+    (void)hipGetLastError();  // a failed allocation only selects the point layout
+    cells = nullptr;
+  }
   (void)hipFree(c->grid_mem);
+  (void)hipFree(c->grid_cells);
   c->grid_mem = mem;
+  c->grid_cells = cells;
   GridC& G = c->grid;
   G = GridC{};
   for (int k = 0; k < 4; ++k) {
@@ -252,8 +309,14 @@ int rapt_grid_set(rapt_ctx* c, int32_t nx, int32_t ny, int32_t nz, int32_t nt, c
       G.ida[k] = 1.0 / h0;
     }
   }
-  G.B = mem + offB;
-  G.E = E ? mem + offE : nullptr;
+  if (cells) {
+    G.brick = 1;
+    G.B = cells;
+    G.E = E ? cells + 32 * ncell : nullptr;
+  } else {
+    G.B = mem + offB;
+    G.E = E ? mem + offE : nullptr;
+  }
   c->grid_set = true;
   if (c->field_set && c->field.model == RAPT_FIELD_GRID) c->field.grid = G;
   return RAPT_OK;

@@ -141,3 +141,21 @@ def test_gridded_uniform_bz_guiding_centres(lpp):
     np.testing.assert_array_equal(a.status, b.status)
     for x, y in zip(a.rows, b.rows):
         np.testing.assert_array_equal(x, y)
+
+
+def test_cell_and_point_layouts_agree(monkeypatch):
+    """The cell-major device copy (the default) and the point-major one
+    (RAPT_GRID_POINTS=1) give bit-identical field values and trajectories."""
+    g1 = _SampledDipole(_axes(False), [0.0, 1.0, 3.0])
+    g2 = _SampledDipole(_axes(False), [0.0, 1.0, 3.0])
+    rng = np.random.default_rng(12)
+    x, y, z = g1._x, g1._y, g1._z
+    n = 300
+    tp = np.c_[rng.uniform(0, 3, n), rng.uniform(x[0], x[-1], n), rng.uniform(y[0], y[-1], n),
+               rng.uniform(z[0], z[-1], n)]
+    ctx = N.context()
+    cells = [ctx.field_eval(g1.descriptor(), w, tp) for w in ("B", "E", "curlb")]
+    monkeypatch.setenv("RAPT_GRID_POINTS", "1")
+    points = [ctx.field_eval(g2.descriptor(), w, tp) for w in ("B", "E", "curlb")]
+    for a, b in zip(cells, points):
+        np.testing.assert_array_equal(a, b)
