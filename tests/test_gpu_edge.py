@@ -1,0 +1,44 @@
+"""Edge cases of the batched entry points: empty batches, one particle per
+layout (a mixed launch of one particle has no pair blocks), and the
+diagnostics on empty inputs."""
+import numpy as np
+import pytest
+
+import rapt_amd as rapt
+from rapt_amd import _native as N
+from rapt_amd import batch
+from rapt_amd import constants as K
+from rapt_amd import fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+def test_empty_batches_return_empty_results():
+    f = F.EarthDipole()
+    p = batch.advance_particles(f, np.zeros((0, 7)), K.m_pr, K.e, 1.0)
+    assert len(p.rows) == 0 and p.status.shape == (0,)
+    g = batch.advance_gcs(f, np.zeros((0, 5)), K.m_el, -K.e, np.zeros(0), np.zeros(0), 0.1, 1.0)
+    assert len(g.rows) == 0 and g.status.shape == (0,)
+    ctx = N.context()
+    out, st = ctx.particle_gc(f.descriptor(), np.zeros((0, 7)), K.m_pr, K.e)
+    assert out.shape == (0, 6) and st.shape == (0,)
+    I, st = ctx.eye(f.descriptor(), np.zeros((0, 4)), np.zeros(0), 50)
+    assert I.shape == (0,) and st.shape == (0,)
+    B = ctx.field_eval(f.descriptor(), "B", np.zeros((0, 4)))
+    assert B.shape == (0, 3)
+
+
+@pytest.mark.parametrize("lpp", [1, 2, 3, 4])
+def test_single_guiding_centre_every_layout(lpp):
+    f = F.DoubleDipole()
+    v = rapt.utils.speedfromKE(1e5, K.m_el)
+    gc = rapt.GuidingCenter([5 * K.Re, 0, 0], v, 40.0, None, 0, K.m_el, -K.e, f)
+    last = gc.trajectory[-1:].copy()
+    prm = dict(rapt.params, solvertolerances=(1e-8, 1e-8))
+    ref = batch.advance_gcs(f, last, K.m_el, -K.e, gc.mu, v, 0.05, 0.5, params=prm,
+                            lanes_per_particle=1)
+    got = batch.advance_gcs(f, last, K.m_el, -K.e, gc.mu, v, 0.05, 0.5, params=prm,
+                            lanes_per_particle=lpp)
+    np.testing.assert_array_equal(got.status, ref.status)
+    np.testing.assert_array_equal(got.rows[0], ref.rows[0])
+    assert ref.status[0] == 1 and len(ref.rows[0]) >= 10
