@@ -531,7 +531,7 @@ def main():
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--mixed-quads", type=int, default=0,
-                    help="with --lpp 3: particles traced on lane quads (0: n/16)")
+                    help="with --lpp 3: particles traced on lane quads (0: n/8)")
     ap.add_argument("--lpp", type=int, default=4, choices=(1, 2, 3, 4),
                     help="lanes per guiding centre (4: a lane quad, 2: a lane pair per "
                          "particle); with --no-tune")

@@ -108,7 +108,7 @@ typedef struct {
                                     first, or by cost after rapt_batch_set_cost); 1:
                                     input order.  Results per particle do not depend
                                     on it */
-    int64_t mixed_quads;         /* lanes_per_particle 3: particles on quads (0: n/16) */
+    int64_t mixed_quads;         /* lanes_per_particle 3: particles on quads (0: n/8) */
 } rapt_trace_desc;
 
 /* Per-particle state, slot-major (state[slot * n + i]).  Filled by the host

@@ -37,7 +37,7 @@ def candidates(mode, stepper, precision, n):
     # mixed: the most expensive particles (the front of the cost order) on
     # quads, the rest on pairs
     if grouped_ok:
-        for frac in (16, 64):
+        for frac in (8, 16):
             out.append(dict(lanes_per_particle=3, block=256, lanes=0,
                             mixed_quads=max(1, n // frac)))
     # fewer resident lanes than particles: refill from the queue sooner

@@ -473,18 +473,18 @@ __device__ __forceinline__ GcK gc_consts(int eom, double charge, double mass, do
 }
 
 // Guiding-centre equations (rapt/GuidingCenter.py:329-395).  Y = (x, y, z, ppar).
-// The algebra, given B, grad|B|, curl b and (non-static TCB only) db/dt at the
-// guiding centre; always FP64.  K = gc_consts(eom, charge, mass, v).
-__device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, bool is_static,
-                                           double charge, double mass, double mu, double v,
-                                           const GcK& K, bool eq, const double* Y, const V3& B,
-                                           const V3& gB, const V3& cb, const V3& db,
-                                           double* out) {
+// The algebra, given B, |B|, b = B/|B| (as mag_unit forms them), grad|B|,
+// curl b and (non-static TCB only) db/dt at the guiding centre; always FP64.
+// K = gc_consts(eom, charge, mass, v).  The TCB Lorentz factor enters only as
+// 1/gamma, formed as 1/sqrt(1 + 2 mu |B| / (m c^2) + (ppar / (m c))^2) by
+// rsqrt_nr.
+__device__ __forceinline__ void gc_algebra_b(int eom, bool has_e, const V3& E, bool is_static,
+                                             double charge, double mass, double mu, double v,
+                                             const GcK& K, bool eq, const double* Y,
+                                             const V3& B, double Bmag, const V3& ub,
+                                             const V3& gB, const V3& cb, const V3& db,
+                                             double* out) {
   const double ppar = Y[3];
-  const double bb = dot3(B, B);
-  const double ibm = rsqrt_nr(bb);
-  const double Bmag = bb * ibm;
-  const V3 ub{B.x * ibm, B.y * ibm, B.z * ibm};
   if (eom == EOM_NT) {
     const double gamma = K.gamv;
     const double gm = gamma * mass;
@@ -512,8 +512,7 @@ __device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, boo
     } else {
       const double imc = K.imc;
       const double pr = ppar * imc;
-      const double gamma = __builtin_sqrt(1.0 + 2.0 * mu * Bmag * (imc * (1.0 / C_LIGHT)) + pr * pr);
-      const double igam = rcp_nr(gamma);
+      const double igam = rsqrt_nr(1.0 + 2.0 * mu * Bmag * (imc * (1.0 / C_LIGHT)) + pr * pr);
       const double mg = mu * igam;
       V3 Es;
       if (has_e) {
@@ -538,6 +537,18 @@ __device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, boo
     out[2] = 0.0;
     out[3] = 0.0;
   }
+}
+// The same algebra given B alone.
+__device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, bool is_static,
+                                           double charge, double mass, double mu, double v,
+                                           const GcK& K, bool eq, const double* Y, const V3& B,
+                                           const V3& gB, const V3& cb, const V3& db,
+                                           double* out) {
+  const double bb = dot3(B, B);
+  const double ibm = rsqrt_nr(bb);
+  const V3 ub{B.x * ibm, B.y * ibm, B.z * ibm};
+  gc_algebra_b(eom, has_e, E, is_static, charge, mass, mu, v, K, eq, Y, B, bb * ibm, ub, gB,
+               cb, db, out);
 }
 
 template <int M>

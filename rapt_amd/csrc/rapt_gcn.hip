@@ -178,6 +178,9 @@ __device__ __forceinline__ CV<1> rhs_gc_quad(const FieldC& F, int eom, double ch
   const double e1 = (q == 0) ? Ba.y : ((q == 1) ? du.y : du.x);
   const double e2 = (q == 0) ? Ba.z : ((q == 3) ? du.y : du.z);
   const V3 B{gb<4, 0>(e0), gb<4, 0>(e1), gb<4, 0>(e2)};
+  // |B| and b at the centre: lane 0's own mag_unit of Ba
+  const double Bm = gb<4, 0>(ma);
+  const V3 ub{gb<4, 0>(ua.x), gb<4, 0>(ua.y), gb<4, 0>(ua.z)};
   const V3 gB{gb<4, 1>(e0), gb<4, 2>(e0), gb<4, 3>(e0)};
   const V3 cb = curl_from_pairs(gb<4, 1>(e1), gb<4, 1>(e2), gb<4, 2>(e1), gb<4, 2>(e2),
                                 gb<4, 3>(e1), gb<4, 3>(e2), i2d);
@@ -194,8 +197,8 @@ __device__ __forceinline__ CV<1> rhs_gc_quad(const FieldC& F, int eom, double ch
   const double Y[4] = {Y0, Y1, Y2, Y3};
   double o[4];
   const V3 Ec = fieldE<M>(F, t, Y0, Y1, Y2);
-  gc_algebra(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y, B,
-             gB, cb, db, o);
+  gc_algebra_b(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y, B, Bm,
+               ub, gB, cb, db, o);
   return CV<1>{{q == 0 ? o[0] : (q == 1 ? o[1] : (q == 2 ? o[2] : o[3]))}};
 }
 
@@ -232,6 +235,9 @@ __device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, double ch
   const double e4 = l0 ? u1.y - u2.y : u2.x - u3.x;
   const double e5 = l0 ? u1.z - u2.z : u2.y - u3.y;
   const V3 B{gb<2, 0>(e0), gb<2, 0>(e1), gb<2, 0>(e2)};
+  // |B| and b at the centre: lane 0's own mag_unit of B0
+  const double Bm = gb<2, 0>(m0);
+  const V3 ub{gb<2, 0>(u0.x), gb<2, 0>(u0.y), gb<2, 0>(u0.z)};
   const V3 gB{gb<2, 0>(e3), gb<2, 1>(e0), gb<2, 1>(e1)};
   const V3 cb = curl_from_pairs(gb<2, 0>(e4), gb<2, 0>(e5), gb<2, 1>(e2), gb<2, 1>(e3),
                                 gb<2, 1>(e4), gb<2, 1>(e5), i2d);
@@ -248,8 +254,8 @@ __device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, double ch
   const double Y[4] = {Y0, Y1, Y2, Y3};
   double o[4];
   const V3 Ec = fieldE<M>(F, t, Y0, Y1, Y2);
-  gc_algebra(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y, B,
-             gB, cb, db, o);
+  gc_algebra_b(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y, B, Bm,
+               ub, gB, cb, db, o);
   return CV<2>{{l0 ? o[0] : o[2], l0 ? o[1] : o[3]}};
 }
 
@@ -290,7 +296,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   double x = 0;
   Y4 y{}, k1{};
   double h = 0, dt = 0, tend = 0, gm = 0, tcur = 0, mass = 0, charge = 0, mu = 0, v = 0;
-  double xend = 0, hmax = 0, facold = 1e-4;
+  double xend = 0, hmax = 0, lfo = LFO0;  // lfo = log2(FACOLD)
   int nstep = 0, nacc_call = 0;
   bool last = false, reject = false, k1ok = false;
   bool drained[2] = {false, false};
@@ -310,7 +316,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   auto begin_call = [&]() {
     xend = x + dt;
     hmax = (A.max_step == 0.0) ? xend - x : A.max_step;
-    facold = 1e-4;
+    lfo = LFO0;
     nstep = 0;
     nacc_call = 0;
     last = false;
@@ -448,7 +454,9 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
     }
     Y4 y1{}, k7{};
     double err = 0.0;
+    double fob = 1.0;
     if (stepping) {
+      fob = ctl_fob(C.beta, lfo);
       using namespace dp5;
       Y4 k2{}, k3{}, k4{}, k5{}, k6{};
 #pragma unroll 1
@@ -506,12 +514,13 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
     if (stepping) {
       bool call_done = false;
       if (q == 0) ++c_stp;
-      const double fac11 = pow(err, C.expo1);
-      double fac = fac11 / pow(facold, C.beta);
+      const double l2e = log2(err);
+      const double fac11 = exp2(C.expo1 * l2e);
+      double fac = fac11 / fob;
       fac = fmax(C.facc2, fmin(C.facc1, fac / C.safe));
       double hnew = h / fac;
       if (err <= 1.0) {
-        facold = fmax(err, 1.0e-4);
+        lfo = fmax(l2e, LFO0);
         ++nacc_call;
         if (q == 0) ++c_acc;
         ++p_acc;

@@ -17,6 +17,22 @@ namespace rapt {
 
 constexpr double UROUND = 2.3e-16;
 
+// The controller's step-size factor err^expo1 / facold^beta in base-2
+// logarithms.  The drivers keep lfo = log2(facold) instead of facold:
+// FACOLD = max(err, 1e-4) of the last accepted step becomes
+// lfo = max(log2(err), LFO0), with lfo = LFO0 = log2(1e-4) at the start of
+// each integrate() call.  One log2 per step attempt and two exp2 replace two
+// pow calls (on gfx950 an FP64 pow is ~220 vector instructions, log2 ~80,
+// exp2 ~32); facold^beta depends on the previous step only, so a driver
+// forms it before the stages, off the chain that ends the step.  The factors
+// match pow to a few ulp (exp2's argument carries the rounding of
+// |expo1 log2(err)| <= ~10); err = 0, inf and NaN give pow's 0, inf and NaN,
+// and beta = 0 gives facold^beta = 1 exactly.
+constexpr double LFO0 = -13.287712379549449;  // log2(1e-4)
+__device__ __forceinline__ double ctl_fob(double beta, double lfo) {
+  return beta == 0.0 ? 1.0 : exp2(beta * lfo);
+}
+
 // Magnitude that scales component i in the error norms.  G = false: |a_i|,
 // scipy's componentwise sc_i = atol + rtol max(|y_i|, |y1_i|).  G = true
 // (FP32-field mode): the largest |a_j| of the component's 3-vector (x, y, z |

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
   // per-lane particle + integrator state
   double x = 0, y[6] = {0, 0, 0, 0, 0, 0}, k1[6] = {0, 0, 0, 0, 0, 0};
   double h = 0, dt = 0, tend = 0, gm = 0, tcur = 0, mass = 0, charge = 0, mu = 0, v = 0;
-  double xend = 0, hmax = 0, facold = 1e-4;
+  double xend = 0, hmax = 0, lfo = LFO0;  // lfo = log2(FACOLD)
   int nstep = 0, nacc_call = 0, stepper = A.stepper;
   bool last = false, reject = false, k1ok = false, gcmode = (KIND == KIND_GC);
   int64_t nrow = 0, rowcap = 0, rowbase = 0;
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
   auto begin_call = [&]() {
     xend = x + dt;
     hmax = (A.max_step == 0.0) ? xend - x : A.max_step;
-    facold = 1e-4;
+    lfo = LFO0;
     nstep = 0;
     nacc_call = 0;
     last = false;
@@ -281,8 +281,10 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
       }
     }
     // ---- one full step attempt --------------------------------------------------
-    double y1[6], k7[6], err = 0.0;
+    double y1[6], k7[6], err = 0.0, fob = 1.0;
     if (stepping) {
+      if (stepper != RAPT_STEP_RKF45)
+        fob = ctl_fob(((stepper == RAPT_STEP_DOP853) ? A.c853 : A.c5).beta, lfo);
       if (!gcmode) {
         LorentzRHS<M, P> f{F, charge, mass, gm, A.eq != 0, 0};
         if (KIND == KIND_ADAPTIVE || stepper == RAPT_STEP_DOP853) {
@@ -337,12 +339,13 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         if (!fail && !(x < xend)) call_done = true;
       } else {
         const CtlParams& C = (stepper == RAPT_STEP_DOP853) ? A.c853 : A.c5;
-        const double fac11 = pow(err, C.expo1);
-        double fac = fac11 / pow(facold, C.beta);
+        const double l2e = log2(err);
+        const double fac11 = exp2(C.expo1 * l2e);
+        double fac = fac11 / fob;
         fac = fmax(C.facc2, fmin(C.facc1, fac / C.safe));
         double hnew = h / fac;
         if (err <= 1.0) {
-          facold = fmax(err, 1.0e-4);
+          lfo = fmax(l2e, LFO0);
           ++nacc_call;
           ++c_acc;
           ++p_acc;

@@ -598,7 +598,7 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   A.order = d->work_order == 0 ? b->d_order : nullptr;
   A.split = b->n;
   if (d->lanes_per_particle == 3) {
-    const int64_t k = d->mixed_quads > 0 ? d->mixed_quads : b->n / 16;
+    const int64_t k = d->mixed_quads > 0 ? d->mixed_quads : b->n / 8;
     A.split = k < b->n ? (k > 0 ? k : 1) : b->n;
   }
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
