@@ -39,11 +39,11 @@ FLOP_RHS_TCB_REFERENCE = 795         # same RHS as the reference writes it (13 f
 PEAK_FP64 = 78.6e12                  # MI355X FP64 vector (and matrix) peak, FLOP/s
 PEAK_HBM = 8.0e12                    # bytes/s
 ROW_BYTES_GC = 40                    # one (t, x, y, z, ppar) output row
-# HBM bytes per launch of the default batch's kernel (k_trace_gcmix, the
-# autotuner's choice), from rocprofv3 FETCH_SIZE and WRITE_SIZE (KiB) scaled by
-# a 256 MiB copy probe (read x2.00, write x1.00);
+# HBM bytes per launch of the default batch's kernel (k_trace_gcmix with 8,192
+# quads, the autotuner's usual choice), from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE (KiB) scaled by a 256 MiB copy probe (read x2.00, write x1.00);
 # profiles/k_trace_gcmix_doubledipole_65536.md has the commands
-PROFILED_TRAFFIC = {65536: (42844.0 * 2.0 + 3068078.34375 * 1.0) * 1024}
+PROFILED_TRAFFIC = {65536: (49621.625 * 2.0 + 3025113.375 * 1.0) * 1024}
 
 
 def log(*a):
@@ -543,7 +543,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", choices=("cost", "dt", "input"), default="cost",
                     help="library work order: descending cost measured by a trial launch "
-                         "over 10%% of the advance (default), shortest output step first, "
+                         "over half the advance (default), shortest output step first, "
                          "or input order")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device copy of this many bytes (HBM counter calibration)")
@@ -586,17 +586,20 @@ def main():
     cost = None
     if args.order == "cost":
         from rapt_amd import autotune as AT
-        cost = AT.trial_cost(w["field"], S, flags, cap, desc)
-        log("work order: descending accepted steps of a 10%%-of-delta trial launch "
-            "(mean %.0f per particle)" % cost.mean())
+        cost = AT.trial_cost(w["field"], S, flags, cap, desc, trial_fraction=0.5)
+        log("work order: descending accepted steps of a trial launch over half the "
+            "advance (mean %.0f per particle)" % cost.mean())
     tuned = None
     if not args.no_tune:
         from rapt_amd import autotune as AT
-        shape, table = AT.tune(w["field"], S, flags, cap, desc, trial_fraction=0.5, log=log,
+        # trials over the whole advance: the mixed layouts differ in how the
+        # longest traces finish, which a shorter trial does not reach
+        shape, table = AT.tune(w["field"], S, flags, cap, desc, trial_fraction=1.0, log=log,
                                cost=cost)
         AT.apply(desc, shape)
         tuned = {"chosen": shape, "trials": [{"shape": t, "steps_per_s": r} for t, r in table],
-                 "trial_fraction_of_delta": 0.5, "trial_work_order": args.order,
+                 "trial_fraction_of_delta": 1.0, "trial_work_order": args.order,
+                 "cost_trial_fraction_of_delta": 0.5 if args.order == "cost" else None,
                  "trial_repeats": 2}
         log("autotune chose %s" % shape)
     b = N.Batch(ctx, n)

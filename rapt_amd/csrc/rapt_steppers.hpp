@@ -220,9 +220,24 @@ constexpr double er1 = 0.1312004499419488073250102996e-01,
                  er12 = -0.2235530786388629525884427845e-01;
 }  // namespace dp8
 
+// c0 k0 + c1 k1 + ... accumulated left to right in fused multiply-adds
+__device__ __forceinline__ double fsum_acc(double s) { return s; }
+template <class... R>
+__device__ __forceinline__ double fsum_acc(double s, double c, double k, R... r) {
+  return fsum_acc(__builtin_fma(c, k, s), r...);
+}
+template <class... R>
+__device__ __forceinline__ double fsum(double c, double k, R... r) {
+  return fsum_acc(c * k, r...);
+}
+
 // One DOP853 step attempt (11 stage evaluations).  Writes the 8th-order
 // solution to y1; returns the combined 5th/3rd-order error norm.  The caller
-// evaluates f(x+h, y1) after acceptance (the 12th evaluation).
+// evaluates f(x+h, y1) after acceptance (the 12th evaluation).  The stage
+// sums y + h (a_1 k_1 + a_2 k_2 + ...) are formed in fused multiply-adds in
+// the order written (the reference's Fortran rounds each product and sum:
+// the results differ by rounding only, and the sums take ~40% fewer
+// instructions, about a fifth of a Lorentz-force step).
 template <int N, bool G = false, class RHS>
 __device__ __forceinline__ double dop853_attempt(RHS& f, double x, double h, const double* y,
                                                  const double* k1, double* y1, double atol,
@@ -230,63 +245,71 @@ __device__ __forceinline__ double dop853_attempt(RHS& f, double x, double h, con
   using namespace dp8;
   double k2[N], k3[N], k4[N], k5[N], k6[N], k7[N], k8[N], k9[N], k10[N], t[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) t[i] = y[i] + h * a21 * k1[i];
+  for (int i = 0; i < N; ++i) t[i] = __builtin_fma(h, a21 * k1[i], y[i]);
   f(x + c2 * h, t, k2);
 #pragma unroll
-  for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
+  for (int i = 0; i < N; ++i) t[i] = __builtin_fma(h, fsum(a31, k1[i], a32, k2[i]), y[i]);
   f(x + c3 * h, t, k3);
 #pragma unroll
-  for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a41 * k1[i] + a43 * k3[i]);
+  for (int i = 0; i < N; ++i) t[i] = __builtin_fma(h, fsum(a41, k1[i], a43, k3[i]), y[i]);
   f(x + c4 * h, t, k4);
 #pragma unroll
-  for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a51 * k1[i] + a53 * k3[i] + a54 * k4[i]);
+  for (int i = 0; i < N; ++i)
+    t[i] = __builtin_fma(h, fsum(a51, k1[i], a53, k3[i], a54, k4[i]), y[i]);
   f(x + c5 * h, t, k5);
 #pragma unroll
-  for (int i = 0; i < N; ++i) t[i] = y[i] + h * (a61 * k1[i] + a64 * k4[i] + a65 * k5[i]);
+  for (int i = 0; i < N; ++i)
+    t[i] = __builtin_fma(h, fsum(a61, k1[i], a64, k4[i], a65, k5[i]), y[i]);
   f(x + c6 * h, t, k6);
 #pragma unroll
   for (int i = 0; i < N; ++i)
-    t[i] = y[i] + h * (a71 * k1[i] + a74 * k4[i] + a75 * k5[i] + a76 * k6[i]);
+    t[i] = __builtin_fma(h, fsum(a71, k1[i], a74, k4[i], a75, k5[i], a76, k6[i]), y[i]);
   f(x + c7 * h, t, k7);
 #pragma unroll
   for (int i = 0; i < N; ++i)
-    t[i] = y[i] + h * (a81 * k1[i] + a84 * k4[i] + a85 * k5[i] + a86 * k6[i] + a87 * k7[i]);
+    t[i] = __builtin_fma(h, fsum(a81, k1[i], a84, k4[i], a85, k5[i], a86, k6[i], a87, k7[i]),
+                         y[i]);
   f(x + c8 * h, t, k8);
 #pragma unroll
   for (int i = 0; i < N; ++i)
-    t[i] = y[i] + h * (a91 * k1[i] + a94 * k4[i] + a95 * k5[i] + a96 * k6[i] + a97 * k7[i] +
-                       a98 * k8[i]);
+    t[i] = __builtin_fma(
+        h, fsum(a91, k1[i], a94, k4[i], a95, k5[i], a96, k6[i], a97, k7[i], a98, k8[i]), y[i]);
   f(x + c9 * h, t, k9);
 #pragma unroll
   for (int i = 0; i < N; ++i)
-    t[i] = y[i] + h * (a101 * k1[i] + a104 * k4[i] + a105 * k5[i] + a106 * k6[i] +
-                       a107 * k7[i] + a108 * k8[i] + a109 * k9[i]);
+    t[i] = __builtin_fma(h,
+                         fsum(a101, k1[i], a104, k4[i], a105, k5[i], a106, k6[i], a107, k7[i],
+                              a108, k8[i], a109, k9[i]),
+                         y[i]);
   f(x + c10 * h, t, k10);
 #pragma unroll
   for (int i = 0; i < N; ++i)
-    t[i] = y[i] + h * (a111 * k1[i] + a114 * k4[i] + a115 * k5[i] + a116 * k6[i] +
-                       a117 * k7[i] + a118 * k8[i] + a119 * k9[i] + a1110 * k10[i]);
+    t[i] = __builtin_fma(h,
+                         fsum(a111, k1[i], a114, k4[i], a115, k5[i], a116, k6[i], a117, k7[i],
+                              a118, k8[i], a119, k9[i], a1110, k10[i]),
+                         y[i]);
   f(x + c11 * h, t, k2);  // stage 11 overwrites k2, as in Hairer's code
   const double xph = x + h;
 #pragma unroll
   for (int i = 0; i < N; ++i)
-    t[i] = y[i] + h * (a121 * k1[i] + a124 * k4[i] + a125 * k5[i] + a126 * k6[i] +
-                       a127 * k7[i] + a128 * k8[i] + a129 * k9[i] + a1210 * k10[i] +
-                       a1211 * k2[i]);
+    t[i] = __builtin_fma(h,
+                         fsum(a121, k1[i], a124, k4[i], a125, k5[i], a126, k6[i], a127, k7[i],
+                              a128, k8[i], a129, k9[i], a1210, k10[i], a1211, k2[i]),
+                         y[i]);
   f(xph, t, k3);  // stage 12 overwrites k3
   double err = 0.0, err2 = 0.0;
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    const double bs = b1 * k1[i] + b6 * k6[i] + b7 * k7[i] + b8 * k8[i] + b9 * k9[i] +
-                      b10 * k10[i] + b11 * k2[i] + b12 * k3[i];
-    y1[i] = y[i] + h * bs;
+    const double bs = fsum(b1, k1[i], b6, k6[i], b7, k7[i], b8, k8[i], b9, k9[i], b10, k10[i],
+                           b11, k2[i], b12, k3[i]);
+    y1[i] = __builtin_fma(h, bs, y[i]);
     const double sk = atol + rtol * fmax(err_mag<N, G>(y, i), err_mag<N, G>(y1, i));
     const double isk = 1.0 / sk;
-    const double e3 = (bs - bhh1 * k1[i] - bhh2 * k9[i] - bhh3 * k3[i]) * isk;
-    err2 += e3 * e3;
-    const double e5 = (er1 * k1[i] + er6 * k6[i] + er7 * k7[i] + er8 * k8[i] + er9 * k9[i] +
-                       er10 * k10[i] + er11 * k2[i] + er12 * k3[i]) * isk;
-    err += e5 * e5;
+    const double e3 = fsum_acc(bs, -bhh1, k1[i], -bhh2, k9[i], -bhh3, k3[i]) * isk;
+    err2 = __builtin_fma(e3, e3, err2);
+    const double e5 = fsum(er1, k1[i], er6, k6[i], er7, k7[i], er8, k8[i], er9, k9[i],
+                           er10, k10[i], er11, k2[i], er12, k3[i]) * isk;
+    err = __builtin_fma(e5, e5, err);
   }
   double deno = err + 0.01 * err2;
   if (deno <= 0.0) deno = 1.0;
