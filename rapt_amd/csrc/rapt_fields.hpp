@@ -253,19 +253,26 @@ __device__ __forceinline__ V3T<T> fieldB(const FieldC& F, T t, T x, T y, T z) {
     }
     return B;
   } else if constexpr (M == F_DOUBLE_DIPOLE) {
-    const T yz2 = y * y + z * z;
-    const T r2a = x * x + yz2;
-    const T xb = x - T(F.p[1]);
-    const T r2b = xb * xb + yz2;
-    const T zz2 = T(2.0) * z * z;
     if constexpr (sizeof(T) == sizeof(double)) {
-      const T ia = inv_r5(r2a);
-      const T ib = T(F.p[2]) * inv_r5(r2b);
-      const T c = T(F.p[0]);
-      return V{c * ((T(3.0) * x * z) * ia + (T(3.0) * xb * z) * ib),
-               c * ((T(3.0) * y * z) * ia + (T(3.0) * y * z) * ib),
-               c * ((zz2 - x * x - y * y) * ia + (zz2 - xb * xb - y * y) * ib)};
+      // B = sum over the two dipoles of c_k |r_k|^-5 (3 x_k z, 3 y z, 3 z^2 - r_k^2)
+      // (2 z^2 - x^2 - y^2 = 3 z^2 - r^2), with the coefficients folded into
+      // the |r|^-5 factors and the sums in fused multiply-adds
+      const double yz2 = __builtin_fma(y, y, z * z);
+      const double r2a = __builtin_fma(x, x, yz2);
+      const double xb = x - F.p[1];
+      const double r2b = __builtin_fma(xb, xb, yz2);
+      const double ca = F.p[0] * inv_r5(r2a);
+      const double cb = (F.p[0] * F.p[2]) * inv_r5(r2b);
+      const double z3 = 3.0 * z;
+      const double zz3 = z3 * z;
+      return V{__builtin_fma(x * z3, ca, (xb * z3) * cb), (y * z3) * (ca + cb),
+               __builtin_fma(zz3 - r2a, ca, (zz3 - r2b) * cb)};
     } else {
+      const T yz2 = y * y + z * z;
+      const T r2a = x * x + yz2;
+      const T xb = x - T(F.p[1]);
+      const T r2b = xb * xb + yz2;
+      const T zz2 = T(2.0) * z * z;
       const T ca = coef_r5(T(F.p[0]), r2a);
       const T cb = coef_r5(T(F.p[0]) * T(F.p[2]), r2b);
       return V{(T(3.0) * x * z) * ca + (T(3.0) * xb * z) * cb,
@@ -461,13 +468,15 @@ __device__ __forceinline__ void rhs_lorentz(const FieldC& F, double charge, doub
 enum Eom : int { EOM_TCB = 0, EOM_BC = 1, EOM_NT = 2 };
 
 // Per-particle constants of the guiding-centre algebra, formed once per
-// particle (or step) instead of once per right-hand side: 1/q, 1/(m c) and
-// the Lorentz factor of the speed v (used by the BC and NT equations only).
+// particle (or step) instead of once per right-hand side: 1/q, 1/(m c),
+// 2 mu / (m c^2) (TCB) and the Lorentz factor of the speed v (BC and NT only).
 struct GcK {
-  double iq, imc, gamv;
+  double iq, imc, kmu, gamv;
 };
-__device__ __forceinline__ GcK gc_consts(int eom, double charge, double mass, double v) {
-  GcK k{rcp_nr(charge), rcp_nr(mass * C_LIGHT), 1.0};
+__device__ __forceinline__ GcK gc_consts(int eom, double charge, double mass, double v,
+                                         double mu) {
+  GcK k{rcp_nr(charge), rcp_nr(mass * C_LIGHT), 0.0, 1.0};
+  k.kmu = 2.0 * mu * (k.imc * (1.0 / C_LIGHT));
   if (eom != EOM_TCB) k.gamv = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
   return k;
 }
@@ -475,9 +484,11 @@ __device__ __forceinline__ GcK gc_consts(int eom, double charge, double mass, do
 // Guiding-centre equations (rapt/GuidingCenter.py:329-395).  Y = (x, y, z, ppar).
 // The algebra, given B, |B|, b = B/|B| (as mag_unit forms them), grad|B|,
 // curl b and (non-static TCB only) db/dt at the guiding centre; always FP64.
-// K = gc_consts(eom, charge, mass, v).  The TCB Lorentz factor enters only as
-// 1/gamma, formed as 1/sqrt(1 + 2 mu |B| / (m c^2) + (ppar / (m c))^2) by
-// rsqrt_nr.
+// K = gc_consts(eom, charge, mass, v, mu).  The TCB Lorentz factor enters only
+// as 1/gamma, formed as 1/sqrt(1 + 2 mu |B| / (m c^2) + (ppar / (m c))^2) by
+// rsqrt_nr.  B*, E* and the TCB velocity are formed in fused multiply-adds
+// with the per-particle factors ppar/q and mu/(gamma q) taken first: the
+// same terms as the reference's, rounded fewer times.
 __device__ __forceinline__ void gc_algebra_b(int eom, bool has_e, const V3& E, bool is_static,
                                              double charge, double mass, double mu, double v,
                                              const GcK& K, bool eq, const double* Y,
@@ -497,7 +508,9 @@ __device__ __forceinline__ void gc_algebra_b(int eom, bool has_e, const V3& E, b
     out[3] = -mu * dot3(ub, gB) / gamma;
   } else {
     const double iq = K.iq;
-    const V3 Bs{B.x + ppar * cb.x * iq, B.y + ppar * cb.y * iq, B.z + ppar * cb.z * iq};
+    const double piq = ppar * iq;
+    const V3 Bs{__builtin_fma(piq, cb.x, B.x), __builtin_fma(piq, cb.y, B.y),
+                __builtin_fma(piq, cb.z, B.z)};
     const double Bspar = dot3(Bs, ub);
     const double iBs = rcp_nr(Bspar);
     if (eom == EOM_BC) {
@@ -510,26 +523,28 @@ __device__ __forceinline__ void gc_algebra_b(int eom, bool has_e, const V3& E, b
       out[2] = (a * Bs.z + bcoef * c.z) * iBs;
       out[3] = -mu * dot3(Bs, gB) / (gamma * Bspar);
     } else {
-      const double imc = K.imc;
-      const double pr = ppar * imc;
-      const double igam = rsqrt_nr(1.0 + 2.0 * mu * Bmag * (imc * (1.0 / C_LIGHT)) + pr * pr);
-      const double mg = mu * igam;
+      const double pr = ppar * K.imc;
+      const double igam = rsqrt_nr(__builtin_fma(pr, pr, __builtin_fma(K.kmu, Bmag, 1.0)));
+      const double mgq = mu * igam * iq;
       V3 Es;
       if (has_e) {
-        Es = V3{E.x - mg * gB.x * iq, E.y - mg * gB.y * iq, E.z - mg * gB.z * iq};
+        Es = V3{__builtin_fma(-mgq, gB.x, E.x), __builtin_fma(-mgq, gB.y, E.y),
+                __builtin_fma(-mgq, gB.z, E.z)};
       } else {
-        Es = V3{-mg * gB.x * iq, -mg * gB.y * iq, -mg * gB.z * iq};
+        Es = V3{-mgq * gB.x, -mgq * gB.y, -mgq * gB.z};
       }
       if (!is_static) {
-        Es.x -= ppar * db.x * iq;
-        Es.y -= ppar * db.y * iq;
-        Es.z -= ppar * db.z * iq;
+        Es.x = __builtin_fma(-piq, db.x, Es.x);
+        Es.y = __builtin_fma(-piq, db.y, Es.y);
+        Es.z = __builtin_fma(-piq, db.z, Es.z);
       }
       const double a = pr * C_LIGHT * igam;
-      const V3 c = cross3(Es, ub);
-      out[0] = (a * Bs.x + c.x) * iBs;
-      out[1] = (a * Bs.y + c.y) * iBs;
-      out[2] = (a * Bs.z + c.z) * iBs;
+      // E* x b
+      const V3 c{__builtin_fma(Es.y, ub.z, -(Es.z * ub.y)), __builtin_fma(Es.z, ub.x, -(Es.x * ub.z)),
+                 __builtin_fma(Es.x, ub.y, -(Es.y * ub.x))};
+      out[0] = __builtin_fma(a, Bs.x, c.x) * iBs;
+      out[1] = __builtin_fma(a, Bs.y, c.y) * iBs;
+      out[2] = __builtin_fma(a, Bs.z, c.z) * iBs;
       out[3] = charge * dot3(Es, Bs) * iBs;
     }
   }

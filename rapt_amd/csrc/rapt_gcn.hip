@@ -378,7 +378,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
         charge = S[RAPT_S_CHARGE * n + pi];
         mu = S[RAPT_S_MU * n + pi];
         v = S[RAPT_S_V * n + pi];
-        gk = gc_consts(A.eom, charge, mass, v);
+        gk = gc_consts(A.eom, charge, mass, v, mu);
         dt = S[RAPT_S_DT * n + pi];
         h = S[RAPT_S_H * n + pi];
         tend = S[RAPT_S_TEND * n + pi];

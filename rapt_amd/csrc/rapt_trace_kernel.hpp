@@ -59,7 +59,7 @@ struct GcRHS {
   double charge, mass, mu, v;
   bool eq;
   int64_t nf;
-  GcK K;  // gc_consts(eom, charge, mass, v), formed once per step
+  GcK K;  // gc_consts(eom, charge, mass, v, mu), formed once per step
   __device__ __forceinline__ void operator()(double t, const double* y, double* o) {
     if constexpr (P == RAPT_PREC_FP32_FIELD) {
       rhs_gc_f32<M>(F, eom, charge, mass, mu, v, K, eq, t, y, o);
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v)};
+        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v, mu)};
         if (stepper != RAPT_STEP_RKF45) f(x, y, k1);
         c_fcn += f.nf;
       }
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v)};
+        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v, mu)};
         const double at = (KIND == KIND_ADAPTIVE) ? A.gc_atol : A.atol;
         const double rt = (KIND == KIND_ADAPTIVE) ? A.gc_rtol : A.rtol;
         h = hinit<4, P == RAPT_PREC_FP32_FIELD>(f, x, y, k1, stepper == RAPT_STEP_DOP853 ? 8 : 5, hmax, at, rt);
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v)};
+        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v, mu)};
         const double at = (KIND == KIND_ADAPTIVE) ? A.gc_atol : A.atol;
         const double rt = (KIND == KIND_ADAPTIVE) ? A.gc_rtol : A.rtol;
         if (KIND == KIND_ADAPTIVE || stepper == RAPT_STEP_DOPRI5) {
