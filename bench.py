@@ -43,7 +43,7 @@ ROW_BYTES_GC = 40                    # one (t, x, y, z, ppar) output row
 # quads, the autotuner's usual choice), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB) scaled by a 256 MiB copy probe (read x2.00, write x1.00);
 # profiles/k_trace_gcmix_doubledipole_65536.md has the commands
-PROFILED_TRAFFIC = {65536: (49621.625 * 2.0 + 3025113.375 * 1.0) * 1024}
+PROFILED_TRAFFIC = {65536: (49471.3125 * 2.0 + 3002504.125 * 1.0) * 1024}
 
 
 def log(*a):
@@ -537,6 +537,9 @@ def main():
                          "particle); with --no-tune")
     ap.add_argument("--block", type=int, default=0,
                     help="threads per block (0: library default); with --no-tune")
+    ap.add_argument("--top", type=int, default=0,
+                    help="trace only this many of the most expensive particles (critical "
+                         "path of the launch); implies --no-cpu")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (use --lpp/--lanes as given)")
     ap.add_argument("--cpu-sample", type=int, default=8192)
@@ -583,6 +586,17 @@ def main():
     desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes, block=args.block,
                        lanes_per_particle=args.lpp, work_order=1 if args.order == "input" else 0,
                        mixed_quads=args.mixed_quads)
+    if args.top:
+        # only the `top` most expensive particles (accepted steps of a full-advance
+        # trial): the launch's critical path without the rest of the batch
+        from rapt_amd import autotune as AT
+        full = AT.trial_cost(w["field"], S, flags, cap, desc, trial_fraction=1.0)
+        idx = np.argsort(-full, kind="stable")[:args.top]
+        S, flags, cap = S[:, idx], flags[idx], cap[idx]
+        n = len(idx)
+        args.no_cpu = True
+        log("top %d particles: %.0f-%.0f accepted steps each" % (n, full[idx].min(),
+                                                                 full[idx].max()))
     cost = None
     if args.order == "cost":
         from rapt_amd import autotune as AT
@@ -660,6 +674,7 @@ def main():
             "lanes": desc.lanes or n * max(desc.lanes_per_particle, 1),
             "lanes_per_particle": desc.lanes_per_particle or 4, "block": desc.block or 256,
             "order": args.order,
+            "top": args.top or None,
         },
         "particles_done": ok,
         "autotune": tuned,
