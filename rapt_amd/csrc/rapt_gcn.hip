@@ -153,7 +153,7 @@ __device__ __forceinline__ StencilOff<L> stencil_offsets(int q, double d) {
 
 // guiding-centre right-hand side, four lanes: component q of the output
 template <int M>
-__device__ __forceinline__ CV<1> rhs_gc_quad(const FieldC& F, int eom, double charge,
+__device__ __forceinline__ CV<1> rhs_gc_quad(const FieldC& F, int eom, bool stat, double charge,
                                              double mass, double mu, double v, const GcK& K,
                                              bool eq, int q, const StencilOff<4>& O, double t,
                                              const CV<1>& yv, int64_t& nf) {
@@ -166,7 +166,7 @@ __device__ __forceinline__ CV<1> rhs_gc_quad(const FieldC& F, int eom, double ch
   const bool ctr = q == 0;
   const V3 Ba = fieldB<M>(F, t, Y0 + O.o[0].x, Y1 + O.o[0].y, Y2 + O.o[0].z);
   double tb = t;
-  if (!F.is_static) tb = ctr ? t - F.tstep : t;
+  if (!stat) tb = ctr ? t - F.tstep : t;
   const V3 Bb = fieldB<M>(F, tb, Y0 + O.o[1].x, Y1 + O.o[1].y, Y2 + O.o[1].z);
   double ma, mb;
   V3 ua, ub_;
@@ -185,7 +185,7 @@ __device__ __forceinline__ CV<1> rhs_gc_quad(const FieldC& F, int eom, double ch
   const V3 cb = curl_from_pairs(gb<4, 1>(e1), gb<4, 1>(e2), gb<4, 2>(e1), gb<4, 2>(e2),
                                 gb<4, 3>(e1), gb<4, 3>(e2), i2d);
   V3 db{0.0, 0.0, 0.0};
-  if (eom == EOM_TCB && !F.is_static) {
+  if (eom == EOM_TCB && !stat) {
     // lane 0's second point was B(t - tstep); B(t + tstep) is a third evaluation
     const V3 Bc = fieldB<M>(F, t + F.tstep, Y0, Y1, Y2);
     const double ic = rsqrt_nr(dot3(Bc, Bc));
@@ -197,14 +197,14 @@ __device__ __forceinline__ CV<1> rhs_gc_quad(const FieldC& F, int eom, double ch
   const double Y[4] = {Y0, Y1, Y2, Y3};
   double o[4];
   const V3 Ec = fieldE<M>(F, t, Y0, Y1, Y2);
-  gc_algebra_b(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y, B, Bm,
+  gc_algebra_b(eom, has_E<M>(), Ec, stat, charge, mass, mu, v, K, eq, Y, B, Bm,
                ub, gB, cb, db, o);
   return CV<1>{{q == 0 ? o[0] : (q == 1 ? o[1] : (q == 2 ? o[2] : o[3]))}};
 }
 
 // guiding-centre right-hand side, two lanes: lane q returns components 2q, 2q+1
 template <int M>
-__device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, double charge,
+__device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, bool stat, double charge,
                                              double mass, double mu, double v, const GcK& K,
                                              bool eq, int q, const StencilOff<2>& O, double t,
                                              const CV<2>& yv, int64_t& nf) {
@@ -215,7 +215,7 @@ __device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, double ch
   const bool l0 = q == 0;
   //   lane 0: centre, +x, -x, centre at t3;  lane 1: +y, -y, +z, -z
   double t3 = t;
-  if (!F.is_static) t3 = l0 ? t - F.tstep : t;
+  if (!stat) t3 = l0 ? t - F.tstep : t;
   const V3 B0 = fieldB<M>(F, t, Y0 + O.o[0].x, Y1 + O.o[0].y, Y2 + O.o[0].z);
   const V3 B1 = fieldB<M>(F, t, Y0 + O.o[1].x, Y1 + O.o[1].y, Y2 + O.o[1].z);
   const V3 B2 = fieldB<M>(F, t, Y0 + O.o[2].x, Y1 + O.o[2].y, Y2 + O.o[2].z);
@@ -242,7 +242,7 @@ __device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, double ch
   const V3 cb = curl_from_pairs(gb<2, 0>(e4), gb<2, 0>(e5), gb<2, 1>(e2), gb<2, 1>(e3),
                                 gb<2, 1>(e4), gb<2, 1>(e5), i2d);
   V3 db{0.0, 0.0, 0.0};
-  if (eom == EOM_TCB && !F.is_static) {
+  if (eom == EOM_TCB && !stat) {
     // lane 0's fourth point was B(t - tstep); B(t + tstep) is a fifth evaluation
     const V3 Bc = fieldB<M>(F, t + F.tstep, Y0, Y1, Y2);
     const double ic = rsqrt_nr(dot3(Bc, Bc));
@@ -254,20 +254,20 @@ __device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, double ch
   const double Y[4] = {Y0, Y1, Y2, Y3};
   double o[4];
   const V3 Ec = fieldE<M>(F, t, Y0, Y1, Y2);
-  gc_algebra_b(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y, B, Bm,
+  gc_algebra_b(eom, has_E<M>(), Ec, stat, charge, mass, mu, v, K, eq, Y, B, Bm,
                ub, gB, cb, db, o);
   return CV<2>{{l0 ? o[0] : o[2], l0 ? o[1] : o[3]}};
 }
 
 template <int M, int L>
-__device__ __forceinline__ CV<4 / L> rhs_gcn(const FieldC& F, int eom, double charge,
+__device__ __forceinline__ CV<4 / L> rhs_gcn(const FieldC& F, int eom, bool stat, double charge,
                                              double mass, double mu, double v, const GcK& K,
                                              bool eq, int q, const StencilOff<L>& O, double t,
                                              const CV<4 / L>& y, int64_t& nf) {
   if constexpr (L == 4) {
-    return rhs_gc_quad<M>(F, eom, charge, mass, mu, v, K, eq, q, O, t, y, nf);
+    return rhs_gc_quad<M>(F, eom, stat, charge, mass, mu, v, K, eq, q, O, t, y, nf);
   } else {
-    return rhs_gc_pair<M>(F, eom, charge, mass, mu, v, K, eq, q, O, t, y, nf);
+    return rhs_gc_pair<M>(F, eom, stat, charge, mass, mu, v, K, eq, q, O, t, y, nf);
   }
 }
 
@@ -279,12 +279,17 @@ __device__ __forceinline__ CV<4 / L> rhs_gcn(const FieldC& F, int eom, double ch
 // The tracer loop of one lane group.  The work queue is split in two: queue
 // 0 holds work items [0, A.split) and queue 1 [A.split, n) (counters
 // A.work[0] and A.work[7]).  A group takes items from queue qown first and
-// from the other queue once its own is drained.
-template <int M, int L>
+// from the other queue once its own is drained.  SP: the instantiation for
+// the Tao-Chan-Brizard equations in a static field, with those two choices
+// compiled in (no runtime branches on them in the right-hand side); the
+// generic one reads them from A.eom and F.is_static.
+template <int M, int L, bool SP>
 __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   constexpr int NC = 4 / L;
   using Y4 = CV<NC>;
   const FieldC& F = A.F;
+  const int eom = SP ? (int)EOM_TCB : A.eom;
+  const bool stat = SP || F.is_static != 0;
   const int64_t n = A.n;
   double* S = A.state;
   const int q = threadIdx.x & (L - 1);
@@ -378,7 +383,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
         charge = S[RAPT_S_CHARGE * n + pi];
         mu = S[RAPT_S_MU * n + pi];
         v = S[RAPT_S_V * n + pi];
-        gk = gc_consts(A.eom, charge, mass, v, mu);
+        gk = gc_consts(eom, charge, mass, v, mu);
         dt = S[RAPT_S_DT * n + pi];
         h = S[RAPT_S_H * n + pi];
         tend = S[RAPT_S_TEND * n + pi];
@@ -407,7 +412,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
     if (__ballot(active || !exhausted) == 0) break;
 
     if (active && !k1ok) {
-      k1 = rhs_gcn<M, L>(F, A.eom, charge, mass, mu, v, gk, eq, q, off, x, y, c_fcn);
+      k1 = rhs_gcn<M, L>(F, eom, stat, charge, mass, mu, v, gk, eq, q, off, x, y, c_fcn);
       k1ok = true;
     }
     if (active && h == 0.0) {
@@ -426,7 +431,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
 #pragma unroll
       for (int i = 0; i < NC; ++i) ye.c[i] = y.c[i] + hh * k1.c[i];
       const Y4 f1 =
-          rhs_gcn<M, L>(F, A.eom, charge, mass, mu, v, gk, eq, q, off, x + hh, ye, c_fcn);
+          rhs_gcn<M, L>(F, eom, stat, charge, mass, mu, v, gk, eq, q, off, x + hh, ye, c_fcn);
 #pragma unroll
       for (int i = 0; i < NC; ++i) r1.c[i] = (f1.c[i] - k1.c[i]) / sk.c[i];
       double der2 = grp_sumsq<L>(r1);
@@ -493,7 +498,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
           case 5: xs = x + c5 * h; break;
           default: xs = x + h; break;
         }
-        const Y4 kk = rhs_gcn<M, L>(F, A.eom, charge, mass, mu, v, gk, eq, q, off, xs, ts, c_fcn);
+        const Y4 kk = rhs_gcn<M, L>(F, eom, stat, charge, mass, mu, v, gk, eq, q, off, xs, ts, c_fcn);
         if (s == 2) k2 = kk;
         else if (s == 3) k3 = kk;
         else if (s == 4) k4 = kk;
@@ -578,10 +583,10 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   }
 }
 
-template <int M, int L>
+template <int M, int L, bool SP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAPT_GCN_WAVES)))
 void k_trace_gcn(KArgs A) {
-  gcn_body<M, L>(A, 0);
+  gcn_body<M, L, SP>(A, 0);
 }
 
 // Mixed layout: blocks [0, quad_blocks) trace with lane quads and take queue
@@ -590,13 +595,13 @@ void k_trace_gcn(KArgs A) {
 // other blocks with lane pairs (fewer instructions per particle-step) and
 // take queue 1 first.  Results do not depend on which layout traced a
 // particle.
-template <int M>
+template <int M, bool SP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAPT_GCN_WAVES)))
 void k_trace_gcmix(KArgs A, int32_t quad_blocks) {
   if ((int32_t)blockIdx.x < quad_blocks) {
-    gcn_body<M, 4>(A, 0);
+    gcn_body<M, 4, SP>(A, 0);
   } else {
-    gcn_body<M, 2>(A, 1);
+    gcn_body<M, 2, SP>(A, 1);
   }
 }
 
@@ -604,11 +609,28 @@ void k_trace_gcmix(KArgs A, int32_t quad_blocks) {
 
 using namespace rapt;
 
+// the TCB + static-field instantiation when it applies (VarEarthDipole is
+// never static)
+template <int M>
+static bool use_sp(const KArgs& A) {
+  if constexpr (M == F_VAR_EARTH_DIPOLE) {
+    return false;
+  } else {
+    return A.eom == EOM_TCB && A.F.is_static != 0;
+  }
+}
+
 template <int M, int L>
 static void launch_gcn_m(hipStream_t st, const KArgs& A, int block, int64_t groups) {
   const int64_t lanes = groups * L;
-  hipLaunchKernelGGL((k_trace_gcn<M, L>), dim3((unsigned)((lanes + block - 1) / block)),
-                     dim3(block), 0, st, A);
+  const dim3 grid((unsigned)((lanes + block - 1) / block));
+  if constexpr (M != F_VAR_EARTH_DIPOLE) {
+    if (use_sp<M>(A)) {
+      hipLaunchKernelGGL((k_trace_gcn<M, L, true>), grid, dim3(block), 0, st, A);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_trace_gcn<M, L, false>), grid, dim3(block), 0, st, A);
 }
 
 template <int L>
@@ -631,8 +653,14 @@ template <int M>
 static void launch_mix_m(hipStream_t st, const KArgs& A, int block, int64_t quads,
                          int64_t pairs) {
   const int64_t qb = (4 * quads + block - 1) / block, pb = (2 * pairs + block - 1) / block;
-  hipLaunchKernelGGL((k_trace_gcmix<M>), dim3((unsigned)(qb + pb)), dim3(block), 0, st, A,
-                     (int32_t)qb);
+  const dim3 grid((unsigned)(qb + pb));
+  if constexpr (M != F_VAR_EARTH_DIPOLE) {
+    if (use_sp<M>(A)) {
+      hipLaunchKernelGGL((k_trace_gcmix<M, true>), grid, dim3(block), 0, st, A, (int32_t)qb);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_trace_gcmix<M, false>), grid, dim3(block), 0, st, A, (int32_t)qb);
 }
 
 // groups resident particle groups of L = 2 or 4 lanes each; L = 3 is the
