@@ -532,7 +532,9 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--mixed-quads", type=int, default=0,
                     help="with --lpp 3: particles traced on lane quads (0: n/8)")
-    ap.add_argument("--lpp", type=int, default=4, choices=(1, 2, 3, 4),
+    ap.add_argument("--mixed-octets", type=int, default=0,
+                    help="with --lpp 5: particles traced on lane octets (0: n/64)")
+    ap.add_argument("--lpp", type=int, default=4, choices=(1, 2, 3, 4, 5, 8),
                     help="lanes per guiding centre (4: a lane quad, 2: a lane pair per "
                          "particle); with --no-tune")
     ap.add_argument("--block", type=int, default=0,
@@ -585,7 +587,7 @@ def main():
     cap = BT._rows_needed(w["delta"], w["dt"])
     desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes, block=args.block,
                        lanes_per_particle=args.lpp, work_order=1 if args.order == "input" else 0,
-                       mixed_quads=args.mixed_quads)
+                       mixed_quads=args.mixed_quads, mixed_octets=args.mixed_octets)
     if args.top:
         # only the `top` most expensive particles (accepted steps of a full-advance
         # trial): the launch's critical path without the rest of the batch

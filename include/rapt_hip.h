@@ -99,16 +99,19 @@ typedef struct {
     int64_t lanes;               /* resident lanes of the persistent grid (0 = auto);
                                     fewer lanes than particles turns on work refill */
     int32_t lanes_per_particle;  /* GC with DOPRI5 in FP64: 4 (default, 0) spreads one
-                                    guiding centre over a lane quad, 2 over a lane pair;
-                                    1 = one lane per particle; 3 = mixed: the first
-                                    mixed_quads particles of the work order on lane
-                                    quads, the rest on lane pairs.  Results do not
-                                    depend on it */
+                                    guiding centre over a lane quad, 2 over a lane pair,
+                                    8 over a lane octet; 1 = one lane per particle;
+                                    3 = mixed: the first mixed_quads particles of the
+                                    work order on lane quads, the rest on lane pairs;
+                                    5 = mixed: the first mixed_octets on octets, the
+                                    next mixed_quads on quads, the rest on pairs.
+                                    Results do not depend on it */
     int32_t work_order;          /* 0: the batch's queue order (shortest output step
                                     first, or by cost after rapt_batch_set_cost); 1:
                                     input order.  Results per particle do not depend
                                     on it */
-    int64_t mixed_quads;         /* lanes_per_particle 3: particles on quads (0: n/8) */
+    int64_t mixed_quads;         /* lanes_per_particle 3, 5: particles on quads (0: n/8) */
+    int64_t mixed_octets;        /* lanes_per_particle 5: particles on octets (0: n/64) */
 } rapt_trace_desc;
 
 /* Per-particle state, slot-major (state[slot * n + i]).  Filled by the host

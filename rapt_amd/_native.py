@@ -54,7 +54,8 @@ class TraceDesc(C.Structure):
                 ("cyclotronresolution", C.c_double), ("gctimestep", C.c_double),
                 ("epss", C.c_double), ("epst", C.c_double), ("delta", C.c_double),
                 ("lanes", C.c_int64), ("lanes_per_particle", C.c_int32),
-                ("work_order", C.c_int32), ("mixed_quads", C.c_int64)]
+                ("work_order", C.c_int32), ("mixed_quads", C.c_int64),
+                ("mixed_octets", C.c_int64)]
 
 
 class Counters(C.Structure):
@@ -429,4 +430,5 @@ def make_desc(mode, params, delta, eom="TaoChanBrizardEOM", stepper=None, check=
     d.lanes_per_particle = int(kw.get("lanes_per_particle", 0))
     d.work_order = int(kw.get("work_order", 0))
     d.mixed_quads = int(kw.get("mixed_quads", 0))
+    d.mixed_octets = int(kw.get("mixed_octets", 0))
     return d

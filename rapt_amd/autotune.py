@@ -1,14 +1,15 @@
 """Launch-shape autotuner for the batched tracers.
 
 The best launch shape depends on the batch: at 65,536 guiding centres a
-grouped kernel (two or four lanes per guiding centre) wins -- one lane per
-particle leaves one wave per SIMD and the slowest particle's chain sets the
-time -- and at 1,048,576 the one-lane kernel wins (enough particles to fill
-the chip, and it spends the fewest issue slots per step).  The tuner times short trial launches of the real batch -- the same
-particles, the same work order, a fraction of the advance -- for each
-candidate (block size, lanes per particle, resident lanes, i.e. the work
-refill threshold) and keeps the one with the highest accepted-steps rate
-(best of two launches each).
+grouped kernel (two, four or eight lanes per guiding centre, or a mix) wins
+-- one lane per particle leaves one wave per SIMD and the slowest particle's
+chain sets the time -- and at 1,048,576 the one-lane kernel wins (enough
+particles to fill the chip, and it spends the fewest issue slots per step).
+The tuner times trial launches of the real batch -- the same particles, the
+same work order, a fraction of the advance -- for each candidate (block
+size, lanes per particle, resident lanes, i.e. the work refill threshold)
+and keeps the one with the highest accepted-steps rate (best of two
+launches each).
 Results are cached per (mode, stepper, field model, precision, equations,
 batch size rounded to a power of two).
 
@@ -34,12 +35,19 @@ def candidates(mode, stepper, precision, n):
             out.append(dict(lanes_per_particle=4, block=block, lanes=0))
             out.append(dict(lanes_per_particle=2, block=block, lanes=0))
         out.append(dict(lanes_per_particle=1, block=block, lanes=0))
+    # eight lanes per particle: the shortest step chain, for batches whose time
+    # is the longest trace's
+    if grouped_ok:
+        out.append(dict(lanes_per_particle=8, block=256, lanes=0))
     # mixed: the most expensive particles (the front of the cost order) on
-    # quads, the rest on pairs
+    # quads -- the very front on octets with lanes_per_particle 5 -- and the
+    # rest on pairs
     if grouped_ok:
         for frac in (8, 16):
             out.append(dict(lanes_per_particle=3, block=256, lanes=0,
                             mixed_quads=max(1, n // frac)))
+        out.append(dict(lanes_per_particle=5, block=256, lanes=0,
+                        mixed_octets=max(1, n // 128), mixed_quads=max(1, n // 8)))
     # fewer resident lanes than particles: refill from the queue sooner
     if n >= 8 * 65536:
         out.append(dict(lanes_per_particle=1, block=256, lanes=4 * 65536))

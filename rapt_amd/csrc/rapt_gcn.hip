@@ -1,28 +1,34 @@
-// rapt_gcn.hip -- GuidingCenter.advance with L = 2 or 4 lanes per guiding centre.
+// rapt_gcn.hip -- GuidingCenter.advance with L = 2, 4 or 8 lanes per guiding centre.
 //
-// A group of L adjacent lanes (inside one DPP quad) carries one particle.
-// Lane g of the group holds NC = 4 / L components of the state (x, y, z,
-// ppar) and of every DOPRI5 stage -- components g*NC .. g*NC+NC-1 -- so the
-// stage combinations are split L ways.  For a right-hand side the group
-// gathers the stage vector with quad-permute DPP moves, then each lane
+// A group of L adjacent lanes carries one particle.  For L = 2 and 4 (inside
+// one DPP quad) lane g of the group holds NC = 4 / L components of the state
+// (x, y, z, ppar) and of every DOPRI5 stage -- components g*NC .. g*NC+NC-1 --
+// so the stage combinations are split L ways; for L = 8 both quads of the
+// group hold the four components as a quad does.  For a right-hand side the
+// group gathers the stage vector with quad-permute DPP moves, then each lane
 // evaluates the field at 8 / L points of the finite-difference stencil:
 //
+//   L = 8: lane 0 the centre, lane k in 1..3 +d on axis k, lane 7 - k -d on
+//          axis k, lane 7 the centre again (row_half_mirror pairs lane i with
+//          lane 7 - i, so each lane swaps its point with its pair's);
 //   L = 4: lane 0 the centre (twice), lane k the +-d pair on axis k;
 //   L = 2: lane 0 the centre, +-x and the centre again; lane 1 +-y and +-z.
 //
-// For a time-dependent field lane 0's spare evaluation is the centre at
-// t - tstep.  Each lane forms its share of grad|B| and curl b, the shares are
-// broadcast back, and every lane runs the shared guiding-centre algebra and
-// keeps its own output components.  All reductions (error norm, HINIT norms)
-// are summed in one fixed order from broadcast values, so the lanes of a
-// group stay bit-identical, take the same branches, and the trajectories are
-// bit-identical to the one-lane kernel's.  Driver semantics are those of
-// k_trace<M, KIND_GC> (rapt_tracer.hip): rapt/GuidingCenter.py:442-458 with
-// scipy's dopri5 call semantics.
+// For a time-dependent field lane 0's spare evaluation (lane 7's for L = 8)
+// is the centre at t - tstep.  Each lane forms its share of grad|B| and
+// curl b, the shares are broadcast back, and every lane runs the shared
+// guiding-centre algebra and keeps its own output components.  All
+// reductions (error norm, HINIT norms) are summed in one fixed order from
+// broadcast values, so the lanes of a group stay bit-identical, take the
+// same branches, and the trajectories are bit-identical to the one-lane
+// kernel's.  Driver semantics are those of k_trace<M, KIND_GC>
+// (rapt_tracer.hip): rapt/GuidingCenter.py:442-458 with scipy's dopri5 call
+// semantics.
 //
-// The two layouts trade issue slots against latency: L = 4 repeats the
-// algebra in four lanes but a group's step is short; L = 2 issues about 40%
-// fewer instructions per particle-step with longer per-lane chains.
+// The layouts trade issue slots against latency: a wide group's step is
+// short (L = 8 evaluates one stencil point per lane) but repeats the algebra
+// in every lane; L = 2 issues about 30% fewer instructions per particle-step
+// than L = 4, with longer per-lane chains.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -58,22 +64,55 @@ template <int L, int I>
 __device__ __forceinline__ int gbi(int v) {
   return __builtin_amdgcn_mov_dpp(v, grp_ctrl<L, I>(), 0xF, 0xF, false);
 }
-template <int L>
-__device__ __forceinline__ long long gbll0(long long v) {
-  const int lo = gbi<L, 0>((int)v), hi = gbi<L, 0>((int)(v >> 32));
-  return ((long long)hi << 32) | (long long)(unsigned)lo;
-}
 
 // this lane's NC components of a four-component vector
 template <int NC>
 struct CV {
   double c[NC];
 };
-
-// the four components of the group's vector, in component order
+// components per lane of an L-lane group (a quad's layout for L = 8)
 template <int L>
-__device__ __forceinline__ void gather4(const CV<4 / L>& v, double out[4]) {
-  if constexpr (L == 4) {
+constexpr int ncomp() {
+  return L >= 4 ? 1 : 4 / L;
+}
+template <int L>
+using CVL = CV<ncomp<L>()>;
+
+// value of lane 7 - i of this lane's half row of eight lanes (row_half_mirror)
+__device__ __forceinline__ double gmir(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x141, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x141, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// lanes 4-7 of each group of eight take the value of lanes 0-3 (row_shr:4
+// written to DPP banks 1 and 3 only); lanes 0-3 keep their own
+__device__ __forceinline__ int gupi(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xA, false);
+}
+__device__ __forceinline__ double gup(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = gupi((int)b), hi = gupi((int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// lane 0's value in every lane of the group
+template <int L>
+__device__ __forceinline__ long long gbll0(long long v) {
+  constexpr int LQ = L == 8 ? 4 : L;
+  int lo = gbi<LQ, 0>((int)v), hi = gbi<LQ, 0>((int)(v >> 32));
+  if constexpr (L == 8) {
+    lo = gupi(lo);
+    hi = gupi(hi);
+  }
+  return ((long long)hi << 32) | (long long)(unsigned)lo;
+}
+
+// the four components of the group's vector, in component order (for L = 8
+// each quad gathers its own copy)
+template <int L>
+__device__ __forceinline__ void gather4(const CVL<L>& v, double out[4]) {
+  if constexpr (L >= 4) {
     out[0] = gb<4, 0>(v.c[0]);
     out[1] = gb<4, 1>(v.c[0]);
     out[2] = gb<4, 2>(v.c[0]);
@@ -87,18 +126,18 @@ __device__ __forceinline__ void gather4(const CV<4 / L>& v, double out[4]) {
 }
 // sum of squares of the four components, in component order
 template <int L>
-__device__ __forceinline__ double grp_sumsq(const CV<4 / L>& v) {
+__device__ __forceinline__ double grp_sumsq(const CVL<L>& v) {
   double a[4];
   gather4<L>(v, a);
   return __builtin_fma(a[3], a[3], __builtin_fma(a[2], a[2], __builtin_fma(a[1], a[1], a[0] * a[0])));
 }
 // 1 when any component of the group's vector is not finite
 template <int L>
-__device__ __forceinline__ bool grp_nonfinite(const CV<4 / L>& v) {
+__device__ __forceinline__ bool grp_nonfinite(const CVL<L>& v) {
   int f = 0;
 #pragma unroll
-  for (int i = 0; i < 4 / L; ++i) f |= __builtin_isfinite(v.c[i]) ? 0 : 1;
-  if constexpr (L == 4) {
+  for (int i = 0; i < ncomp<L>(); ++i) f |= __builtin_isfinite(v.c[i]) ? 0 : 1;
+  if constexpr (L >= 4) {
     return (gbi<4, 0>(f) | gbi<4, 1>(f) | gbi<4, 2>(f) | gbi<4, 3>(f)) != 0;
   } else {
     return (gbi<2, 0>(f) | gbi<2, 1>(f)) != 0;
@@ -125,7 +164,16 @@ template <int L>
 __device__ __forceinline__ StencilOff<L> stencil_offsets(int q, double d) {
   const V3 ctr{-0.0, -0.0, -0.0};
   StencilOff<L> s;
-  if constexpr (L == 4) {
+  if constexpr (L == 8) {
+    // lanes 0 and 7: the centre; lane k in 1..3: +d on axis k; lane 7 - k: -d
+    if (q == 0 || q == 7) {
+      s.o[0] = ctr;
+    } else if (q < 4) {
+      s.o[0] = V3{q == 1 ? d : 0.0, q == 2 ? d : 0.0, q == 3 ? d : 0.0};
+    } else {
+      s.o[0] = V3{q == 6 ? -d : 0.0, q == 5 ? -d : 0.0, q == 4 ? -d : 0.0};
+    }
+  } else if constexpr (L == 4) {
     // lane 0: the centre twice; lane k: +d and -d on axis k
     if (q == 0) {
       s.o[0] = ctr;
@@ -259,12 +307,68 @@ __device__ __forceinline__ CV<2> rhs_gc_pair(const FieldC& F, int eom, bool stat
   return CV<2>{{l0 ? o[0] : o[2], l0 ? o[1] : o[3]}};
 }
 
+// guiding-centre right-hand side, eight lanes: component q & 3 of the output.
+// Both quads hold the state, so each gathers it with quad moves.  Every lane
+// evaluates one stencil point and swaps |B| and b with its mirror lane, which
+// gives lanes 0-3 exactly the pairs a lane quad evaluates itself (lane k: +d
+// and -d on axis k; lane 0: the centre and, for a time-dependent field, the
+// centre at t - tstep); the lower quad runs the quad's broadcasts and
+// algebra, and the upper quad takes its output.
+template <int M>
+__device__ __forceinline__ CV<1> rhs_gc_oct(const FieldC& F, int eom, bool stat, double charge,
+                                            double mass, double mu, double v, const GcK& K,
+                                            bool eq, int q, const StencilOff<8>& O, double t,
+                                            const CV<1>& yv, int64_t& nf) {
+  ++nf;
+  const double yq = yv.c[0];
+  const double Y0 = gb<4, 0>(yq), Y1 = gb<4, 1>(yq), Y2 = gb<4, 2>(yq), Y3 = gb<4, 3>(yq);
+  const double i2d = F.i2d;
+  double ta = t;
+  if (!stat) ta = q == 7 ? t - F.tstep : t;
+  const V3 Ba = fieldB<M>(F, ta, Y0 + O.o[0].x, Y1 + O.o[0].y, Y2 + O.o[0].z);
+  double ma;
+  V3 ua;
+  mag_unit(Ba, ma, ua);
+  const double mb = gmir(ma);
+  const V3 ub_{gmir(ua.x), gmir(ua.y), gmir(ua.z)};
+  const V3 du{ua.x - ub_.x, ua.y - ub_.y, ua.z - ub_.z};
+  const double g = (ma - mb) * i2d;
+  const double e0 = (q == 0) ? Ba.x : g;
+  const double e1 = (q == 0) ? Ba.y : ((q == 1) ? du.y : du.x);
+  const double e2 = (q == 0) ? Ba.z : ((q == 3) ? du.y : du.z);
+  const V3 B{gb<4, 0>(e0), gb<4, 0>(e1), gb<4, 0>(e2)};
+  const double Bm = gb<4, 0>(ma);
+  const V3 ub{gb<4, 0>(ua.x), gb<4, 0>(ua.y), gb<4, 0>(ua.z)};
+  const V3 gB{gb<4, 1>(e0), gb<4, 2>(e0), gb<4, 3>(e0)};
+  const V3 cb = curl_from_pairs(gb<4, 1>(e1), gb<4, 1>(e2), gb<4, 2>(e1), gb<4, 2>(e2),
+                                gb<4, 3>(e1), gb<4, 3>(e2), i2d);
+  V3 db{0.0, 0.0, 0.0};
+  if (eom == EOM_TCB && !stat) {
+    // lane 0's partner point was B(t - tstep); B(t + tstep) is a second evaluation
+    const V3 Bc = fieldB<M>(F, t + F.tstep, Y0, Y1, Y2);
+    const double ic = rsqrt_nr(dot3(Bc, Bc));
+    const double s = F.ihts;
+    const double l0 = (Bc.x * ic - ub_.x) * s, l1 = (Bc.y * ic - ub_.y) * s,
+                 l2 = (Bc.z * ic - ub_.z) * s;
+    db = V3{gb<4, 0>(l0), gb<4, 0>(l1), gb<4, 0>(l2)};
+  }
+  const double Y[4] = {Y0, Y1, Y2, Y3};
+  double o[4];
+  const V3 Ec = fieldE<M>(F, t, Y0, Y1, Y2);
+  gc_algebra_b(eom, has_E<M>(), Ec, stat, charge, mass, mu, v, K, eq, Y, B, Bm, ub, gB, cb, db,
+               o);
+  const int qc = q & 3;
+  return CV<1>{{gup(qc == 0 ? o[0] : (qc == 1 ? o[1] : (qc == 2 ? o[2] : o[3])))}};
+}
+
 template <int M, int L>
-__device__ __forceinline__ CV<4 / L> rhs_gcn(const FieldC& F, int eom, bool stat, double charge,
-                                             double mass, double mu, double v, const GcK& K,
-                                             bool eq, int q, const StencilOff<L>& O, double t,
-                                             const CV<4 / L>& y, int64_t& nf) {
-  if constexpr (L == 4) {
+__device__ __forceinline__ CVL<L> rhs_gcn(const FieldC& F, int eom, bool stat, double charge,
+                                          double mass, double mu, double v, const GcK& K,
+                                          bool eq, int q, const StencilOff<L>& O, double t,
+                                          const CVL<L>& y, int64_t& nf) {
+  if constexpr (L == 8) {
+    return rhs_gc_oct<M>(F, eom, stat, charge, mass, mu, v, K, eq, q, O, t, y, nf);
+  } else if constexpr (L == 4) {
     return rhs_gc_quad<M>(F, eom, stat, charge, mass, mu, v, K, eq, q, O, t, y, nf);
   } else {
     return rhs_gc_pair<M>(F, eom, stat, charge, mass, mu, v, K, eq, q, O, t, y, nf);
@@ -276,16 +380,19 @@ __device__ __forceinline__ CV<4 / L> rhs_gcn(const FieldC& F, int eom, bool stat
 #define RAPT_GCN_WAVES 2
 #endif
 
-// The tracer loop of one lane group.  The work queue is split in two: queue
-// 0 holds work items [0, A.split) and queue 1 [A.split, n) (counters
-// A.work[0] and A.work[7]).  A group takes items from queue qown first and
-// from the other queue once its own is drained.  SP: the instantiation for
+// The tracer loop of one lane group.  The work queue is split in three:
+// queue 0 holds work items [0, A.split), queue 1 [A.split, A.split2) and
+// queue 2 [A.split2, n) (counters A.work[0], A.work[6], A.work[7]; a single
+// layout has everything in queue 0).  A group takes items from queue qown
+// first and from the others, in queue order, once its own is drained; the
+// queues hold descending costs, so that is the longest work left.  SP: the
+// instantiation for
 // the Tao-Chan-Brizard equations in a static field, with those two choices
 // compiled in (no runtime branches on them in the right-hand side); the
 // generic one reads them from A.eom and F.is_static.
 template <int M, int L, bool SP>
 __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
-  constexpr int NC = 4 / L;
+  constexpr int NC = ncomp<L>();
   using Y4 = CV<NC>;
   const FieldC& F = A.F;
   const int eom = SP ? (int)EOM_TCB : A.eom;
@@ -293,7 +400,8 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   const int64_t n = A.n;
   double* S = A.state;
   const int q = threadIdx.x & (L - 1);
-  const int c0 = q * NC;  // first component this lane holds
+  const int c0 = (L == 8 ? (q & 3) : q) * NC;  // first component this lane holds
+  const bool owner = L < 8 || q < 4;           // the lane stores its components
   const StencilOff<L> off = stencil_offsets<L>(q, F.gstep);
   GcK gk{};
   bool active = false, exhausted = false;
@@ -304,7 +412,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   double xend = 0, hmax = 0, lfo = LFO0;  // lfo = log2(FACOLD)
   int nstep = 0, nacc_call = 0;
   bool last = false, reject = false, k1ok = false;
-  bool drained[2] = {false, false};
+  bool drained[3] = {false, false, false};
   int64_t nrow = 0, rowcap = 0, rowbase = 0;
   int64_t c_acc = 0, c_rej = 0, c_stp = 0, c_fcn = 0, p_acc = 0;
   const int W = 5;
@@ -314,8 +422,10 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   auto write_row = [&](double trow) {
     double* r = A.rows + (rowbase + nrow) * W;
     if (q == 0) r[0] = trow;
+    if (owner) {
 #pragma unroll
-    for (int i = 0; i < NC; ++i) r[1 + c0 + i] = y.c[i];
+      for (int i = 0; i < NC; ++i) r[1 + c0 + i] = y.c[i];
+    }
     ++nrow;
   };
   auto begin_call = [&]() {
@@ -328,8 +438,10 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
     reject = false;
   };
   auto finish = [&](int32_t code) {
+    if (owner) {
 #pragma unroll
-    for (int i = 0; i < NC; ++i) S[(1 + c0 + i) * n + pi] = y.c[i];
+      for (int i = 0; i < NC; ++i) S[(1 + c0 + i) * n + pi] = y.c[i];
+    }
     if (q == 0) {
       S[0 * n + pi] = x;
       S[RAPT_S_MU * n + pi] = mu;
@@ -352,11 +464,13 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
       int64_t got = n;
       if (q == 0) {
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          const int qq = k == 0 ? qown : 1 - qown;
+        for (int k = 0; k < 3; ++k) {
+          // qown, then the other two in queue order
+          const int qq = k == 0 ? qown : (k == 1 ? (qown == 0 ? 1 : 0) : (qown == 2 ? 1 : 2));
           if (got < n || drained[qq]) continue;
-          const int64_t lo = qq == 0 ? 0 : A.split, hi = qq == 0 ? A.split : n;
-          const int64_t t = (int64_t)atomicAdd(&A.work[qq == 0 ? 0 : 7], 1ULL);
+          const int64_t lo = qq == 0 ? 0 : (qq == 1 ? A.split : A.split2);
+          const int64_t hi = qq == 0 ? A.split : (qq == 1 ? A.split2 : n);
+          const int64_t t = (int64_t)atomicAdd(&A.work[qq == 0 ? 0 : (qq == 1 ? 6 : 7)], 1ULL);
           if (t < hi - lo) {
             got = lo + t;
           } else {
@@ -589,19 +703,22 @@ void k_trace_gcn(KArgs A) {
   gcn_body<M, L, SP>(A, 0);
 }
 
-// Mixed layout: blocks [0, quad_blocks) trace with lane quads and take queue
-// 0 first (the most expensive particles in cost order: their step chains
-// are the launch's critical path, and a quad's step is the shortest), the
-// other blocks with lane pairs (fewer instructions per particle-step) and
-// take queue 1 first.  Results do not depend on which layout traced a
-// particle.
+// Mixed layout: blocks [0, oct_blocks) trace with lane octets and take
+// queue 0 first, blocks [oct_blocks, wide_blocks) with lane quads and take
+// queue 1 first (the most expensive particles in cost order: their step
+// chains are the launch's critical path, and a wide group's step is the
+// shortest), the other blocks with lane pairs (fewer instructions per
+// particle-step) and take queue 2 first.  Results do not depend on which
+// layout traced a particle.
 template <int M, bool SP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RAPT_GCN_WAVES)))
-void k_trace_gcmix(KArgs A, int32_t quad_blocks) {
-  if ((int32_t)blockIdx.x < quad_blocks) {
-    gcn_body<M, 4, SP>(A, 0);
+void k_trace_gcmix(KArgs A, int32_t oct_blocks, int32_t wide_blocks) {
+  if ((int32_t)blockIdx.x < oct_blocks) {
+    gcn_body<M, 8, SP>(A, 0);
+  } else if ((int32_t)blockIdx.x < wide_blocks) {
+    gcn_body<M, 4, SP>(A, 1);
   } else {
-    gcn_body<M, 2, SP>(A, 1);
+    gcn_body<M, 2, SP>(A, 2);
   }
 }
 
@@ -650,38 +767,44 @@ static void launch_gcn_l(hipStream_t st, int model, const KArgs& A, int block, i
 }
 
 template <int M>
-static void launch_mix_m(hipStream_t st, const KArgs& A, int block, int64_t quads,
-                         int64_t pairs) {
-  const int64_t qb = (4 * quads + block - 1) / block, pb = (2 * pairs + block - 1) / block;
-  const dim3 grid((unsigned)(qb + pb));
+static void launch_mix_m(hipStream_t st, const KArgs& A, int block, int64_t octets,
+                         int64_t quads, int64_t pairs) {
+  const int64_t ob = (8 * octets + block - 1) / block, qb = (4 * quads + block - 1) / block,
+                pb = (2 * pairs + block - 1) / block;
+  const dim3 grid((unsigned)(ob + qb + pb));
   if constexpr (M != F_VAR_EARTH_DIPOLE) {
     if (use_sp<M>(A)) {
-      hipLaunchKernelGGL((k_trace_gcmix<M, true>), grid, dim3(block), 0, st, A, (int32_t)qb);
+      hipLaunchKernelGGL((k_trace_gcmix<M, true>), grid, dim3(block), 0, st, A, (int32_t)ob,
+                         (int32_t)(ob + qb));
       return;
     }
   }
-  hipLaunchKernelGGL((k_trace_gcmix<M, false>), grid, dim3(block), 0, st, A, (int32_t)qb);
+  hipLaunchKernelGGL((k_trace_gcmix<M, false>), grid, dim3(block), 0, st, A, (int32_t)ob,
+                     (int32_t)(ob + qb));
 }
 
-// groups resident particle groups of L = 2 or 4 lanes each; L = 3 is the
-// mixed layout with `groups` quads and `pairs` pairs
+// Single layouts (L = 2, 4, 8): `groups` resident groups of L lanes.  Mixed
+// layouts (lanes_per_particle 3, 5): `groups` octets, `quads` quads and
+// `pairs` pairs.
 void rapt_launch_gcn(hipStream_t st, int model, int lanes_per_particle, const KArgs& A,
-                     int block, int64_t groups, int64_t pairs) {
-  if (lanes_per_particle == 3) {
+                     int block, int64_t groups, int64_t quads, int64_t pairs) {
+  if (lanes_per_particle == 3 || lanes_per_particle == 5) {
     switch (model) {
-      case 0: launch_mix_m<0>(st, A, block, groups, pairs); break;
-      case 1: launch_mix_m<1>(st, A, block, groups, pairs); break;
-      case 2: launch_mix_m<2>(st, A, block, groups, pairs); break;
-      case 3: launch_mix_m<3>(st, A, block, groups, pairs); break;
-      case 4: launch_mix_m<4>(st, A, block, groups, pairs); break;
-      case 5: launch_mix_m<5>(st, A, block, groups, pairs); break;
-      case 6: launch_mix_m<6>(st, A, block, groups, pairs); break;
-      case 7: launch_mix_m<7>(st, A, block, groups, pairs); break;
-      case 8: launch_mix_m<8>(st, A, block, groups, pairs); break;
-      case 9: launch_mix_m<9>(st, A, block, groups, pairs); break;
+      case 0: launch_mix_m<0>(st, A, block, groups, quads, pairs); break;
+      case 1: launch_mix_m<1>(st, A, block, groups, quads, pairs); break;
+      case 2: launch_mix_m<2>(st, A, block, groups, quads, pairs); break;
+      case 3: launch_mix_m<3>(st, A, block, groups, quads, pairs); break;
+      case 4: launch_mix_m<4>(st, A, block, groups, quads, pairs); break;
+      case 5: launch_mix_m<5>(st, A, block, groups, quads, pairs); break;
+      case 6: launch_mix_m<6>(st, A, block, groups, quads, pairs); break;
+      case 7: launch_mix_m<7>(st, A, block, groups, quads, pairs); break;
+      case 8: launch_mix_m<8>(st, A, block, groups, quads, pairs); break;
+      case 9: launch_mix_m<9>(st, A, block, groups, quads, pairs); break;
     }
   } else if (lanes_per_particle == 2) {
     launch_gcn_l<2>(st, model, A, block, groups);
+  } else if (lanes_per_particle == 8) {
+    launch_gcn_l<8>(st, model, A, block, groups);
   } else {
     launch_gcn_l<4>(st, model, A, block, groups);
   }

@@ -42,9 +42,11 @@ struct KArgs {
   const int64_t* row_off;
   const int64_t* row_cap;
   double* rows;
-  unsigned long long* work;  // [0] next work item, [1..6] counters
+  unsigned long long* work;  // [0] next work item, [1..5] counters; grouped GC
+                             // kernels: [0], [6], [7] next item of queues 0, 1, 2
   const int64_t* order;      // work item -> particle index (nullptr: identity)
-  int64_t split;             // grouped GC kernels: items [0, split) form queue 0
+  int64_t split, split2;     // grouped GC kernels: work items [0, split) form queue 0,
+                             // [split, split2) queue 1, [split2, n) queue 2
 };
 
 }  // namespace rapt

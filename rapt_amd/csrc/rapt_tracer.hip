@@ -140,7 +140,7 @@ static CtlParams ctl(double safety, double dfactor, double ifactor, double beta,
 }
 
 void rapt_launch_gcn(hipStream_t st, int model, int lanes_per_particle, const KArgs& A,
-                     int block, int64_t groups, int64_t pairs);
+                     int block, int64_t groups, int64_t quads, int64_t pairs);
 
 void rapt_launch_trace_f32(hipStream_t st, int model, int kind, const KArgs& A, int block,
                            int64_t lanes);
@@ -526,9 +526,10 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
     return set_err(c, RAPT_E_ARG, "PARTICLE mode steps with DOP853 or RKF45");
   if (d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOP853)
     return set_err(c, RAPT_E_ARG, "GC mode steps with DOPRI5 or RKF45");
-  if (d->lanes_per_particle < 0 || d->lanes_per_particle > 4)
-    return set_err(c, RAPT_E_ARG, "lanes_per_particle must be 0-4");
-  if (d->mixed_quads < 0) return set_err(c, RAPT_E_ARG, "mixed_quads must be >= 0");
+  if (d->lanes_per_particle < 0 || (d->lanes_per_particle > 5 && d->lanes_per_particle != 8))
+    return set_err(c, RAPT_E_ARG, "lanes_per_particle must be 0-5 or 8");
+  if (d->mixed_quads < 0 || d->mixed_octets < 0)
+    return set_err(c, RAPT_E_ARG, "mixed_quads and mixed_octets must be >= 0");
   if (d->work_order != 0 && d->work_order != 1)
     return set_err(c, RAPT_E_ARG, "work_order must be 0 or 1");
   if (d->precision != RAPT_PREC_FP64 && d->precision != RAPT_PREC_FP32_FIELD)
@@ -596,10 +597,23 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   A.rows = b->d_rows;
   A.work = b->d_work;
   A.order = d->work_order == 0 ? b->d_order : nullptr;
+  // work queues of the grouped kernels: one queue [0, n) for a single layout;
+  // for the mixed layouts [0, split) octets, [split, split2) quads and
+  // [split2, n) pairs (lanes_per_particle 3: no octets, at least one quad)
   A.split = b->n;
-  if (d->lanes_per_particle == 3) {
-    const int64_t k = d->mixed_quads > 0 ? d->mixed_quads : b->n / 8;
-    A.split = k < b->n ? (k > 0 ? k : 1) : b->n;
+  A.split2 = b->n;
+  if (d->lanes_per_particle == 3 || d->lanes_per_particle == 5) {
+    const int64_t n = b->n;
+    int64_t ko = 0;
+    if (d->lanes_per_particle == 5) {
+      ko = d->mixed_octets > 0 ? d->mixed_octets : n / 64;
+      ko = ko < 1 ? 1 : (ko > n ? n : ko);
+    }
+    int64_t kq = d->mixed_quads > 0 ? d->mixed_quads : n / 8;
+    const int64_t kq_min = d->lanes_per_particle == 3 ? 1 : 0;
+    kq = kq < kq_min ? kq_min : (kq > n - ko ? n - ko : kq);
+    A.split = ko;
+    A.split2 = ko + kq;
   }
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
   if (lanes > b->n) lanes = b->n;
@@ -617,14 +631,23 @@ static void launch_tracer(rapt_batch* b, const rapt_trace_desc* d, const KArgs& 
                           int64_t lanes, bool grouped) {
   rapt_ctx* c = b->ctx;
   if (grouped) {
-    if (d->lanes_per_particle == 3) {
-      // every particle resident: A.split quads, the rest pairs
-      rapt_launch_gcn(c->stream, c->field.model, 3, A, block, A.split, b->n - A.split);
+    if (d->lanes_per_particle == 3 || d->lanes_per_particle == 5) {
+      // octets and quads for their queues; pairs for the rest, all resident
+      // unless d->lanes caps the resident lanes (all groups' together): then
+      // fewer pair groups refill from the queues
+      const int64_t octets = A.split, quads = A.split2 - A.split;
+      int64_t pairs = b->n - A.split2;
+      if (d->lanes > 0) {
+        const int64_t p = (d->lanes - 8 * octets - 4 * quads) / 2;
+        pairs = p < 1 ? 1 : (p < pairs ? p : pairs);
+      }
+      rapt_launch_gcn(c->stream, c->field.model, d->lanes_per_particle, A, block, octets, quads,
+                      pairs);
     } else {
-      const int L = d->lanes_per_particle == 2 ? 2 : 4;
+      const int L = d->lanes_per_particle == 2 ? 2 : (d->lanes_per_particle == 8 ? 8 : 4);
       int64_t groups = d->lanes > 0 ? (d->lanes + L - 1) / L : b->n;
       if (groups > b->n) groups = b->n;
-      rapt_launch_gcn(c->stream, c->field.model, L, A, block, groups, 0);
+      rapt_launch_gcn(c->stream, c->field.model, L, A, block, groups, 0, 0);
     }
   } else if (d->precision == RAPT_PREC_FP32_FIELD) {
     const int kind = d->mode == RAPT_MODE_PARTICLE ? KIND_PARTICLE

@@ -28,7 +28,7 @@ def test_empty_batches_return_empty_results():
     assert B.shape == (0, 3)
 
 
-@pytest.mark.parametrize("lpp", [1, 2, 3, 4])
+@pytest.mark.parametrize("lpp", [1, 2, 3, 4, 5, 8])
 def test_single_guiding_centre_every_layout(lpp):
     f = F.DoubleDipole()
     v = rapt.utils.speedfromKE(1e5, K.m_el)

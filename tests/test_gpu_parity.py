@@ -279,7 +279,7 @@ def _gc_batch(field, n, seed_pa=None):
     ("crossedEB", lambda: F.UniformCrossedEB(Ey=1e-3, Bz=1e-7), "TaoChanBrizardEOM"),
     ("dipunif", lambda: F.DipolePlusUniform(), "TaoChanBrizardEOM"),
 ])
-@pytest.mark.parametrize("lpp", [4, 2, 3])
+@pytest.mark.parametrize("lpp", [4, 2, 3, 8, 5])
 def test_gc_grouped_lanes_match_single_lane(name, field, eom, lpp):
     """Four lanes per guiding centre (the default), two lanes, and the mixed
     layout (3: the front of the work order on quads, the rest on pairs)
@@ -303,7 +303,7 @@ def test_gc_grouped_lanes_match_single_lane(name, field, eom, lpp):
     np.testing.assert_array_equal(b.state, a.state)
 
 
-@pytest.mark.parametrize("lpp", [4, 2, 3])
+@pytest.mark.parametrize("lpp", [4, 2, 3, 8, 5])
 def test_gc_grouped_refill_and_adiabaticity_check(lpp):
     """Two- and four-lane kernels with fewer resident groups than guiding
     centres (work refill) and the adiabaticity check on: bit-identical rows
@@ -321,7 +321,7 @@ def test_gc_grouped_refill_and_adiabaticity_check(lpp):
         np.testing.assert_array_equal(y, x)
 
 
-@pytest.mark.parametrize("lpp", [1, 2, 3, 4])
+@pytest.mark.parametrize("lpp", [1, 2, 3, 4, 5, 8])
 def test_work_order_does_not_change_results(lpp):
     """The library's work order (shortest output step first, the default)
     and input order give bit-identical per-particle results."""
@@ -420,7 +420,7 @@ def test_autotuner_picks_a_candidate_and_keeps_results():
         np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("lpp", [4, 3, 2, 1])
+@pytest.mark.parametrize("lpp", [4, 3, 2, 1, 5])
 def test_output_epochs_reproduce_one_launch(lpp):
     """A run in hipGraph-replayed output epochs of 7 rows per particle gives
     each particle's rows of one uninterrupted launch, epoch by epoch (every
@@ -456,7 +456,7 @@ def test_output_epochs_reproduce_one_launch(lpp):
     np.testing.assert_array_equal(status, one.status)
 
 
-@pytest.mark.parametrize("lpp", [4, 3, 2, 1])
+@pytest.mark.parametrize("lpp", [4, 3, 2, 1, 5])
 def test_cost_work_order_does_not_change_results(lpp):
     """A work order set by cost (Batch.set_cost: here a trial launch's step
     counts and a random permutation, with refill) gives the rows, states and

@@ -35,7 +35,7 @@ def test_library_exports_header_symbols():
 def test_struct_layouts_match_header():
     # field descriptor: 2 int32 + 2 double + 16 double
     assert ctypes.sizeof(N.FieldDesc) == 8 + 16 + 128
-    assert ctypes.sizeof(N.TraceDesc) == 8 * 4 + 8 * 18 + 8 + 8 + 8
+    assert ctypes.sizeof(N.TraceDesc) == 8 * 4 + 8 * 18 + 8 + 8 + 8 + 8
     assert ctypes.sizeof(N.Counters) == 7 * 8
 
 
