@@ -74,15 +74,18 @@ using V3 = V3T<double>;
 using V3f = V3T<float>;
 
 // 1/sqrt(x) and 1/x from the hardware estimates (v_rsq_f64, v_rcp_f64) with
-// two Newton steps each: full double precision to within an ulp or two, in 9
-// and 5 dependent operations instead of the IEEE sqrt and divide sequences.
-// Arguments are positive and normal here (squared lengths, field magnitudes).
+// two Newton steps each: full double precision to within an ulp or two, in 8
+// and 5 operations instead of the IEEE sqrt and divide sequences.  A step of
+// rsqrt_nr is y + y (1/2 - (x/2) y y) with x/2 formed once: the same values as
+// y + (y/2)(1 - x y y), since halving is exact.  Arguments are positive and
+// normal here (squared lengths, field magnitudes).
 __device__ __forceinline__ double rsqrt_nr(double x) {
+  const double hx = 0.5 * x;
   double y = __builtin_amdgcn_rsq(x);
-  double e = __builtin_fma(-(x * y), y, 1.0);
-  y = __builtin_fma(0.5 * y, e, y);
-  e = __builtin_fma(-(x * y), y, 1.0);
-  return __builtin_fma(0.5 * y, e, y);
+  double e = __builtin_fma(-(hx * y), y, 0.5);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-(hx * y), y, 0.5);
+  return __builtin_fma(y, e, y);
 }
 __device__ __forceinline__ double rcp_nr(double x) {
   double r = __builtin_amdgcn_rcp(x);
