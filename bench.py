@@ -43,7 +43,7 @@ ROW_BYTES_GC = 40                    # one (t, x, y, z, ppar) output row
 # quads, the autotuner's usual choice), from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE (KiB) scaled by a 256 MiB copy probe (read x2.00, write x1.00);
 # profiles/k_trace_gcmix_doubledipole_65536.md has the commands
-PROFILED_TRAFFIC = {65536: (49471.3125 * 2.0 + 3002504.125 * 1.0) * 1024}
+PROFILED_TRAFFIC = {65536: (44261.8125 * 2.0 + 3032515.625 * 1.0) * 1024}
 
 
 def log(*a):
