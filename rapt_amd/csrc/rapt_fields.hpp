@@ -570,16 +570,16 @@ __device__ __forceinline__ void gc_algebra(int eom, bool has_e, const V3& E, boo
 }
 
 template <int M>
-__device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, double charge, double mass,
-                                       double mu, double v, const GcK& K, bool eq, double t,
-                                       const double* Y, double* out) {
+// stat: the field is static (F.is_static, or a compile-time true)
+__device__ __forceinline__ void rhs_gc(const FieldC& F, int eom, bool stat, double charge,
+                                       double mass, double mu, double v, const GcK& K, bool eq,
+                                       double t, const double* Y, double* out) {
   const V3 B = fieldB<M>(F, t, Y[0], Y[1], Y[2]);
   V3 gB, cb, db{0.0, 0.0, 0.0};
   grad_curl<M>(F, t, Y[0], Y[1], Y[2], gB, cb);
-  if (eom == EOM_TCB && !F.is_static) db = dbdt<M>(F, t, Y[0], Y[1], Y[2]);
+  if (eom == EOM_TCB && !stat) db = dbdt<M>(F, t, Y[0], Y[1], Y[2]);
   const V3 Ec = fieldE<M>(F, t, Y[0], Y[1], Y[2]);
-  gc_algebra(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y,
-             B, gB, cb, db, out);
+  gc_algebra(eom, has_E<M>(), Ec, stat, charge, mass, mu, v, K, eq, Y, B, gB, cb, db, out);
 }
 
 // ---- FP32-field precision mode ----------------------------------------------------
@@ -638,14 +638,13 @@ __device__ __forceinline__ void gc_fields_f32(const FieldC& F, double t, double 
 }
 
 template <int M>
-__device__ __forceinline__ void rhs_gc_f32(const FieldC& F, int eom, double charge, double mass,
-                                           double mu, double v, const GcK& K, bool eq, double t,
-                                           const double* Y, double* out) {
+__device__ __forceinline__ void rhs_gc_f32(const FieldC& F, int eom, bool stat, double charge,
+                                           double mass, double mu, double v, const GcK& K,
+                                           bool eq, double t, const double* Y, double* out) {
   V3 B, gB, cb, db;
-  gc_fields_f32<M>(F, t, Y[0], Y[1], Y[2], B, gB, cb, db, eom == EOM_TCB && !F.is_static);
+  gc_fields_f32<M>(F, t, Y[0], Y[1], Y[2], B, gB, cb, db, eom == EOM_TCB && !stat);
   const V3 Ec = fieldE<M>(F, t, Y[0], Y[1], Y[2]);
-  gc_algebra(eom, has_E<M>(), Ec, F.is_static != 0, charge, mass, mu, v, K, eq, Y,
-             B, gB, cb, db, out);
+  gc_algebra(eom, has_E<M>(), Ec, stat, charge, mass, mu, v, K, eq, Y, B, gB, cb, db, out);
 }
 
 // Lorentz force with the field and the force in float; gamma*m in double.

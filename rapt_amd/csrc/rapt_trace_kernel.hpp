@@ -52,7 +52,8 @@ struct LorentzRHS {
   }
 };
 
-template <int M, int P>
+// SP: the Tao-Chan-Brizard equations in a static field, compiled in
+template <int M, int P, bool SP>
 struct GcRHS {
   const FieldC& F;
   int eom;
@@ -61,18 +62,24 @@ struct GcRHS {
   int64_t nf;
   GcK K;  // gc_consts(eom, charge, mass, v, mu), formed once per step
   __device__ __forceinline__ void operator()(double t, const double* y, double* o) {
+    const int e = SP ? (int)EOM_TCB : eom;
+    const bool stat = SP || F.is_static != 0;
     if constexpr (P == RAPT_PREC_FP32_FIELD) {
-      rhs_gc_f32<M>(F, eom, charge, mass, mu, v, K, eq, t, y, o);
+      rhs_gc_f32<M>(F, e, stat, charge, mass, mu, v, K, eq, t, y, o);
     } else {
-      rhs_gc<M>(F, eom, charge, mass, mu, v, K, eq, t, y, o);
+      rhs_gc<M>(F, e, stat, charge, mass, mu, v, K, eq, t, y, o);
     }
     ++nf;
   }
 };
 
 // ---- the persistent tracer kernel ---------------------------------------------------
-template <int M, int KIND, int P>
+// SP: for KIND_GC, the instantiation for the Tao-Chan-Brizard equations in a
+// static field (those choices compiled in); the generic one reads A.eom and
+// F.is_static.
+template <int M, int KIND, int P, bool SP = false>
 __global__ __launch_bounds__(256) void k_trace(KArgs A) {
+  const int eom = SP ? (int)EOM_TCB : A.eom;
   const FieldC& F = A.F;
   const int64_t n = A.n;
   double* S = A.state;
@@ -238,7 +245,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v, mu)};
+        GcRHS<M, P, SP> f{F, eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(eom, charge, mass, v, mu)};
         if (stepper != RAPT_STEP_RKF45) f(x, y, k1);
         c_fcn += f.nf;
       }
@@ -252,7 +259,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v, mu)};
+        GcRHS<M, P, SP> f{F, eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(eom, charge, mass, v, mu)};
         const double at = (KIND == KIND_ADAPTIVE) ? A.gc_atol : A.atol;
         const double rt = (KIND == KIND_ADAPTIVE) ? A.gc_rtol : A.rtol;
         h = hinit<4, P == RAPT_PREC_FP32_FIELD>(f, x, y, k1, stepper == RAPT_STEP_DOP853 ? 8 : 5, hmax, at, rt);
@@ -295,7 +302,7 @@ __global__ __launch_bounds__(256) void k_trace(KArgs A) {
         gm = f.gm;
         c_fcn += f.nf;
       } else {
-        GcRHS<M, P> f{F, A.eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(A.eom, charge, mass, v, mu)};
+        GcRHS<M, P, SP> f{F, eom, charge, mass, mu, v, A.eq != 0, 0, gc_consts(eom, charge, mass, v, mu)};
         const double at = (KIND == KIND_ADAPTIVE) ? A.gc_atol : A.atol;
         const double rt = (KIND == KIND_ADAPTIVE) ? A.gc_rtol : A.rtol;
         if (KIND == KIND_ADAPTIVE || stepper == RAPT_STEP_DOPRI5) {
@@ -511,6 +518,13 @@ __global__ __launch_bounds__(256) void k_field_eval(FieldC F, int what, int64_t 
     case 11: o[0] = F.is_static ? __builtin_nan("") : timescale<M>(F, t, x, y, z); break;
     default: break;
   }
+}
+
+// the one-lane GC kernel's TCB + static-field instantiation applies
+// (VarEarthDipole is never static)
+template <int M>
+static inline bool trace_use_sp(const KArgs& A) {
+  return M != F_VAR_EARTH_DIPOLE && A.eom == EOM_TCB && A.F.is_static != 0;
 }
 
 }  // namespace rapt

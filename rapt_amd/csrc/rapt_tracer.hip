@@ -1,8 +1,8 @@
 // rapt_tracer.hip -- the C-ABI of librapt_hip.so (include/rapt_hip.h): contexts,
 // field evaluation, batches, and the launch of the FP64 tracer kernels.  The
 // kernel itself is k_trace in rapt_trace_kernel.hpp; the FP32-field
-// instantiations live in rapt_tracer_f32.hip and the two- and four-lane
-// guiding-centre kernels in rapt_gcn.hip.
+// instantiations live in rapt_tracer_f32.hip and the grouped guiding-centre
+// kernels (two, four or eight lanes per particle) in rapt_gcn.hip.
 #include <hip/hip_runtime.h>
 
 #include <stdarg.h>
@@ -148,6 +148,13 @@ void rapt_launch_trace_f32(hipStream_t st, int model, int kind, const KArgs& A, 
 template <int M, int K>
 static void launch_trace(rapt_ctx* c, const KArgs& A, int block, int64_t lanes) {
   const int64_t g = (lanes + block - 1) / block;
+  if constexpr (K == KIND_GC && M != F_VAR_EARTH_DIPOLE) {
+    if (trace_use_sp<M>(A)) {
+      hipLaunchKernelGGL((k_trace<M, K, RAPT_PREC_FP64, true>), dim3((unsigned)g), dim3(block),
+                         0, c->stream, A);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_trace<M, K, RAPT_PREC_FP64>), dim3((unsigned)g), dim3(block), 0,
                      c->stream, A);
 }
@@ -618,7 +625,7 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
   if (lanes > b->n) lanes = b->n;
   if (lanes < 1) lanes = 1;
-  // the two- and four-lane kernels are FP64 only
+  // the grouped kernels (two, four or eight lanes, mixed) are FP64 only
   const bool grouped = d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOPRI5 &&
                     d->lanes_per_particle != 1 && d->precision == RAPT_PREC_FP64;
   lanes_out = lanes;

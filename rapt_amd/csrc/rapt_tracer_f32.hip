@@ -12,6 +12,13 @@ using namespace rapt;
 template <int M, int K>
 static void launch_f32(hipStream_t st, const KArgs& A, int block, int64_t lanes) {
   const int64_t g = (lanes + block - 1) / block;
+  if constexpr (K == KIND_GC && M != F_VAR_EARTH_DIPOLE) {
+    if (trace_use_sp<M>(A)) {
+      hipLaunchKernelGGL((k_trace<M, K, RAPT_PREC_FP32_FIELD, true>), dim3((unsigned)g),
+                         dim3(block), 0, st, A);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_trace<M, K, RAPT_PREC_FP32_FIELD>), dim3((unsigned)g), dim3(block), 0,
                      st, A);
 }
