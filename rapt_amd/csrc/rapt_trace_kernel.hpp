@@ -77,8 +77,18 @@ struct GcRHS {
 // SP: for KIND_GC, the instantiation for the Tao-Chan-Brizard equations in a
 // static field (those choices compiled in); the generic one reads A.eom and
 // F.is_static.
+// Register budget: two waves per SIMD (256 registers per lane) for the
+// analytic fields, where it costs 12-92 bytes of spills per lane in the GC
+// and Particle kernels and hides the dependent-chain latency better (one-lane
+// GC +21% FP64, +31% FP32-field on 1,048,576 particles); one wave for
+// gridded fields, whose interpolation would spill over a kilobyte per lane.
+template <int M>
+constexpr int trace_waves() {
+  return M == F_GRID ? 1 : 2;
+}
 template <int M, int KIND, int P, bool SP = false>
-__global__ __launch_bounds__(256) void k_trace(KArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(trace_waves<M>())))
+void k_trace(KArgs A) {
   const int eom = SP ? (int)EOM_TCB : A.eom;
   const FieldC& F = A.F;
   const int64_t n = A.n;
