@@ -9,7 +9,7 @@ namespace rapt {
 // ---- adiabaticity tests ------------------------------------------------------------
 // Particle.isadiabatic (rapt/Particle.py:345-384) on the row (t, r, p)
 template <int M>
-__device__ bool particle_isadiabatic(const FieldC& F, const KArgs& A, double t, const double* y,
+__device__ __forceinline__ bool particle_isadiabatic(const FieldC& F, const KArgs& A, double t, const double* y,
                                      double mass, double charge) {
   const V3 p{y[3], y[4], y[5]};
   const double gm = __builtin_sqrt(mass * mass + dot3(p, p) / C_LIGHT_SQ);
@@ -32,7 +32,7 @@ __device__ bool particle_isadiabatic(const FieldC& F, const KArgs& A, double t, 
 // GuidingCenter.isadiabatic (rapt/GuidingCenter.py:287-327); the cycper()
 // term keeps the reference's unnormalised ppar**2 (rapt/GuidingCenter.py:535)
 template <int M>
-__device__ bool gc_isadiabatic(const FieldC& F, const KArgs& A, double t, const double* y,
+__device__ __forceinline__ bool gc_isadiabatic(const FieldC& F, const KArgs& A, double t, const double* y,
                                double mass, double charge, double mu) {
   const double Bmag = magB<M>(F, t, y[0], y[1], y[2]);
   const double pp = y[3];
@@ -72,7 +72,7 @@ __device__ bool gc_isadiabatic(const FieldC& F, const KArgs& A, double t, const 
 // (rapt/GuidingCenter.py:168-186, __init__ :123-133).  y: Particle state in,
 // GC state out (x, y, z, ppar); returns false when the iteration fails.
 template <int M>
-__device__ bool particle_to_gc(const FieldC& F, double t, double* y, double mass, double charge,
+__device__ __forceinline__ bool particle_to_gc(const FieldC& F, double t, double* y, double mass, double charge,
                                double& mu, double& vout) {
   const V3 p{y[3], y[4], y[5]};
   const double gm = __builtin_sqrt(mass * mass + dot3(p, p) / C_LIGHT_SQ);
@@ -118,7 +118,7 @@ __device__ bool particle_to_gc(const FieldC& F, double t, double* y, double mass
 // (rapt/utils.py:376-433) at gyrophase 0; the field is read at t = 0 there,
 // the new Particle's tcur.  y: GC state in, Particle state out.
 template <int M>
-__device__ void gc_to_particle(const FieldC& F, double t, double* y, double mass, double charge,
+__device__ __forceinline__ void gc_to_particle(const FieldC& F, double t, double* y, double mass, double charge,
                                double mu) {
   (void)t;
   const double Bm = magB<M>(F, t, y[0], y[1], y[2]);
