@@ -216,7 +216,7 @@ __device__ bool brent(Fn& f, double a, double b, double& root) {
 // trace points below 70 degrees, a rule whose discretisation error depends on
 // where the trace points fall and so adds jitter to finite differences of I.
 template <int M>
-__device__ int32_t line_integrals(const BounceArgs& A, int64_t lane, double t, double x0,
+__device__ __forceinline__ int32_t line_integrals(const BounceArgs& A, int64_t lane, double t, double x0,
                                   double y0, double z0, double Bm, double* sb_out,
                                   double* eye_out, double* lat_out) {
   const FieldC& F = A.F;
@@ -466,7 +466,7 @@ struct DriftRHS {
   double Bm, coef, dstep;
   int32_t st;
   int64_t nf;
-  __device__ void operator()(double t, const double* Y, double* out) {
+  __device__ __forceinline__ void operator()(double t, const double* Y, double* out) {
     ++nf;
     const FieldC& F = A.F;
     const V3 B = fieldB<M>(F, t, Y[0], Y[1], Y[2]);
