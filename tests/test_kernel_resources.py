@@ -2,7 +2,7 @@
 
 Reads the AMDGPU code-object metadata of rapt_amd/lib/librapt_hip.so (the
 gfx950 objects in its .hip_fatbin section, one offload bundle per source
-file) with the ROCm LLVM tools, without running anything on a GPU.
+file), parsed here in Python: nothing runs on a GPU and no tool is started.
 
 What it guards: when a device function that receives the kernel's by-value
 arguments (or a pointer to a local array) is left out of line, the compiler
@@ -16,59 +16,79 @@ would not.
 """
 import os
 import re
-import shutil
-import subprocess
+import struct
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "rapt_amd", "lib", "librapt_hip.so")
-LLVM = "/opt/rocm/llvm/bin"
-BUNDLER = os.path.join(LLVM, "clang-offload-bundler")
-READELF = os.path.join(LLVM, "llvm-readelf")
-TARGET = "hipv4-amdgcn-amd-amdhsa--gfx950"
+TARGET = b"hipv4-amdgcn-amd-amdhsa--gfx950"
 MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
+NT_AMDGPU_METADATA = 32
 
 
-def _private_sizes(tmp):
-    fat = os.path.join(tmp, "fat.bin")
-    subprocess.run(["objcopy", "--dump-section", ".hip_fatbin=" + fat, LIB], check=True)
-    data = open(fat, "rb").read()
-    starts = [m.start() for m in re.finditer(re.escape(MAGIC), data)]
+def _sections(elf):
+    """(name, type, offset, size) of every section of a little-endian ELF64."""
+    assert elf[:4] == b"\x7fELF" and elf[4] == 2 and elf[5] == 1
+    shoff, = struct.unpack_from("<Q", elf, 0x28)
+    shentsize, shnum, shstrndx = struct.unpack_from("<HHH", elf, 0x3A)
+    raw = [struct.unpack_from("<IIQQQQ", elf, shoff + i * shentsize) for i in range(shnum)]
+    strtab = raw[shstrndx][4]
+    out = []
+    for name, typ, _flags, _addr, off, size in raw:
+        end = elf.index(b"\0", strtab + name)
+        out.append((elf[strtab + name:end].decode(), typ, off, size))
+    return out
+
+
+def _kernel_metadata(code_object):
+    """The amdhsa.kernels list of an AMDGPU code object's metadata note."""
+    import msgpack
+    for _name, typ, off, size in _sections(code_object):
+        if typ != 7:  # SHT_NOTE
+            continue
+        p, end = off, off + size
+        while p + 12 <= end:
+            namesz, descsz, ntype = struct.unpack_from("<III", code_object, p)
+            name = code_object[p + 12:p + 12 + namesz].rstrip(b"\0")
+            d0 = p + 12 + ((namesz + 3) & ~3)
+            if name == b"AMDGPU" and ntype == NT_AMDGPU_METADATA:
+                return msgpack.unpackb(code_object[d0:d0 + descsz], raw=False)["amdhsa.kernels"]
+            p = d0 + ((descsz + 3) & ~3)
+    raise AssertionError("no AMDGPU metadata note")
+
+
+def _private_sizes(lib_path):
+    """Kernel name -> private segment bytes per lane, over every gfx950 code
+    object in the library's .hip_fatbin section (one offload bundle per
+    source file: magic, entry count, then offset, size, triple per entry)."""
+    data = open(lib_path, "rb").read()
+    fat = [(off, size) for name, _t, off, size in _sections(data) if name == ".hip_fatbin"]
+    assert fat, "no .hip_fatbin section"
+    off, size = fat[0]
+    fb = data[off:off + size]
+    starts = [m.start() for m in re.finditer(re.escape(MAGIC), fb)]
     assert starts, "no offload bundle in .hip_fatbin"
     sizes = {}
-    for i, s in enumerate(starts):
-        e = starts[i + 1] if i + 1 < len(starts) else len(data)
-        b = os.path.join(tmp, "b%d.bin" % i)
-        o = os.path.join(tmp, "d%d.o" % i)
-        with open(b, "wb") as f:
-            f.write(data[s:e])
-        subprocess.run([BUNDLER, "--type=o", "--targets=" + TARGET, "--input=" + b,
-                        "--output=" + o, "--unbundle"], check=True)
-        notes = subprocess.run([READELF, "--notes", o], capture_output=True, text=True,
-                               check=True).stdout
-        name = None
-        for line in notes.splitlines():
-            m = re.match(r"\s+\.name:\s+(\S+)", line)
-            if m:
-                name = m.group(1)
-            m = re.match(r"\s+\.private_segment_fixed_size:\s+(\d+)", line)
-            if m and name:
-                sizes[name] = int(m.group(1))
-                name = None
+    for s in starts:
+        n, = struct.unpack_from("<Q", fb, s + len(MAGIC))
+        q = s + len(MAGIC) + 8
+        for _ in range(n):
+            eoff, esize, tlen = struct.unpack_from("<QQQ", fb, q)
+            triple = fb[q + 24:q + 24 + tlen]
+            q += 24 + tlen
+            if triple == TARGET:
+                for k in _kernel_metadata(fb[s + eoff:s + eoff + esize]):
+                    sizes[k[".name"]] = int(k[".private_segment_fixed_size"])
     return sizes
 
 
 @pytest.fixture(scope="module")
-def sizes(tmp_path_factory):
+def sizes():
     if not os.path.exists(LIB):
         pytest.skip("librapt_hip.so is not built")
-    for tool in (BUNDLER, READELF):
-        if not os.path.exists(tool):
-            pytest.skip(tool + " is not installed")
-    if shutil.which("objcopy") is None:
-        pytest.skip("objcopy is not installed")
-    return _private_sizes(str(tmp_path_factory.mktemp("kmeta")))
+    pytest.importorskip("msgpack")
+    return _private_sizes(LIB)
 
 
 def test_every_kernel_has_metadata(sizes):
