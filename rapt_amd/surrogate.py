@@ -8,7 +8,7 @@ latitude [deg]).  Labels come from the traced bounce quantities on the GPU
 (rapt_bounce_labels: BounceCenter's drift, rapt/BounceCenter.py:235-244, and
 flutils.bounceperiod, rapt/flutils.py:232-252).  At inference, queries on
 which the members' relative spread exceeds a threshold are traced instead,
-so the traced method is both the fallback and the judge of the surrogate.
+so the traced method is both the fallback and the yardstick of the surrogate.
 
 Training and inference run on the device (rapt_amd/csrc/rapt_mlp.hip); the
 host only shuffles sample indices and holds the arrays.
