@@ -156,7 +156,8 @@ def run_particle_dipole(args):
     S = BT.particle_state(last7, rapt.m_pr, rapt.e)
     dt = BT.particle_dt(f, last7, rapt.m_pr, rapt.e, 20)
     cap = BT._rows_needed(10.0, dt)
-    desc = N.make_desc(N.MODE_PARTICLE, dict(rapt.params), 10.0, lanes=args.lanes)
+    desc = N.make_desc(N.MODE_PARTICLE, dict(rapt.params), 10.0, lanes=args.lanes,
+                       block=args.block)
     tot, el, out = timed_run(f, N.MODE_PARTICLE, S, np.zeros(n, np.int32), cap, desc,
                              args.steps, args.warmup, 7)
     value = tot["naccpt"] / el
