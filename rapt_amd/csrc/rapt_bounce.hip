@@ -612,16 +612,18 @@ __global__ __launch_bounds__(256) void k_bounce_center(BounceArgs A, BCArgs C) {
 
 using namespace rapt;
 
+// 64-lane blocks: a run holds at most 16,384 lanes (256 waves), which 64-lane
+// blocks spread over every CU instead of a quarter of them.
 template <int M>
 static void launch_hb(rapt_ctx* c, const BounceArgs& A) {
-  const int B = 256;
+  const int B = 64;
   hipLaunchKernelGGL(k_halfbounce<M>, dim3((unsigned)((A.lanes + B - 1) / B)), dim3(B), 0,
                      c->stream, A);
 }
 
 template <int M>
 static void launch_lab(rapt_ctx* c, const BounceArgs& A, const LabelArgs& L) {
-  const int B = 256;
+  const int B = 64;
   hipLaunchKernelGGL(k_bounce_labels<M>, dim3((unsigned)((A.lanes + B - 1) / B)), dim3(B), 0,
                      c->stream, A, L);
 }
