@@ -86,7 +86,7 @@ typedef struct {
     int32_t nsteps;              /* scipy nsteps (500) */
     int32_t enforce_equatorial;  /* params["enforce equatorial"] */
     int32_t check_adiabaticity;  /* raise Adiabatic / NonAdiabatic -> stop (PARTICLE/GC) */
-    int32_t block;               /* threads per block (0 = default: 64 for one-lane runs of
+    int32_t block;               /* threads per block (0 = default: 64 for launches of
                                     fewer than four waves per CU, else 256) */
     double rtol, atol;           /* params["solvertolerances"] */
     double safety, dfactor, ifactor, beta;  /* scipy dop853/dopri5 knobs */

@@ -627,17 +627,30 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   // the grouped kernels (two, four or eight lanes, mixed) are FP64 only
   const bool grouped = d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOPRI5 &&
                     d->lanes_per_particle != 1 && d->precision == RAPT_PREC_FP64;
-  // Default block: 256 lanes, except for the one-lane kernels on a batch of
-  // fewer than four waves per CU, which take 64 so that the waves spread
-  // over every CU (a CU's waves share its scalar unit and instruction
-  // fetch): 1,024 Particle-mode protons advance in 39-41 ms instead of 52,
-  // 16,384 in 48 instead of 63; at 65,536 (four waves per CU) the two are
-  // equal for the analytic dipole and 64 was slower on a gridded one.
+  // Default block: 256 lanes, except for launches of fewer than four waves
+  // per CU, which take 64 so that the waves spread over every CU (a CU's
+  // waves share its scalar unit and instruction fetch): 1,024 Particle-mode
+  // protons advance in 39-43 ms instead of 52, 16,384 in 48 instead of 63;
+  // at 65,536 (four waves per CU) the two are equal for the analytic dipole
+  // and 64 was slower on a gridded one.  The grouped kernels count the lanes
+  // of all their groups.
   int block = d->block > 0 ? d->block : 0;
   if (block == 0) {
+    int64_t launched = lanes;
+    if (grouped) {
+      const int lpp = d->lanes_per_particle;
+      if (lpp == 3 || lpp == 5) {
+        launched = 8 * A.split + 4 * (A.split2 - A.split) + 2 * (b->n - A.split2);
+        if (d->lanes > 0 && d->lanes < launched) launched = d->lanes;
+      } else {
+        const int L = lpp == 2 ? 2 : (lpp == 8 ? 8 : 4);
+        int64_t groups = d->lanes > 0 ? (d->lanes + L - 1) / L : b->n;
+        launched = (groups < b->n ? groups : b->n) * L;
+      }
+    }
     int cus = 0;
     HIP_OK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    block = (!grouped && lanes < (int64_t)4 * 64 * cus) ? 64 : 256;
+    block = launched < (int64_t)4 * 64 * cus ? 64 : 256;
   }
   block_out = block;
   lanes_out = lanes;
