@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "librapt_oracle.so")
+# RAPT_ORACLE_LIB names another build of the oracle (e.g. one with a planted
+# mistake, to see that the pins fail)
+_LIB_PATH = os.environ.get("RAPT_ORACLE_LIB") or os.path.join(_HERE, "librapt_oracle.so")
 
 ZERO, EARTH_DIPOLE, DOUBLE_DIPOLE, UNIFORM_BZ, UNIFORM_CROSSED_EB, VAR_EARTH_DIPOLE, \
     PARABOLIC, CURRENT_SHEET, DIPOLE_UNIFORM = range(9)
@@ -72,6 +74,23 @@ def lib():
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                        C.POINTER(Counters)]
         L.orc_adaptive_run.restype = C.c_int64
+        L.orc_adaptive_continue.argtypes = [C.POINTER(Field), C.POINTER(Params), C.c_int, dp,
+                                            C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, dp, C.c_int64, C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_int32), C.POINTER(Counters)]
+        L.orc_adaptive_continue.restype = C.c_int64
+        L.orc_particle_isadiabatic.argtypes = [C.POINTER(Field), C.POINTER(Params), dp,
+                                               C.c_double, C.c_double]
+        L.orc_particle_isadiabatic.restype = C.c_int
+        L.orc_gc_isadiabatic.argtypes = [C.POINTER(Field), C.POINTER(Params), dp, C.c_double,
+                                         C.c_double, C.c_double]
+        L.orc_gc_isadiabatic.restype = C.c_int
+        L.orc_adaptive_to_gc.argtypes = [C.POINTER(Field), dp, C.c_double, C.c_double, dp, dp,
+                                         dp]
+        L.orc_adaptive_to_gc.restype = C.c_int
+        L.orc_adaptive_to_particle.argtypes = [C.POINTER(Field), dp, C.c_double, C.c_double,
+                                               C.c_double, dp]
+        L.orc_adaptive_to_particle.restype = None
         L.orc_rkf.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, dp,
                               C.c_double, C.c_double, C.c_double, dp, dp, C.c_int64,
                               C.POINTER(C.c_int64)]
@@ -225,6 +244,53 @@ def adaptive_run(field, params, pos, vel, t0, mass, charge, delta, cap=400000):
                                C.byref(cnt))
     return rows[:n].copy(), dict(nswitch=nsw.value, status=st.value, naccpt=cnt.naccpt,
                                  nrejct=cnt.nrejct, nfcn=cnt.nfcn)
+
+
+def adaptive_continue(field, params, mode, last, mu, v, mass, charge, delta, cap=400000):
+    """A further Adaptive.advance(delta) from the last tracer (mode 0 Particle
+    with last row (t, x, y, z, px, py, pz); mode 1 GuidingCenter with last row
+    (t, x, y, z, ppar), moment mu, speed v)."""
+    last = np.ascontiguousarray(last, dtype=np.float64)
+    rows = np.zeros((cap, 8))
+    nsw = C.c_int32()
+    st = C.c_int32()
+    cnt = Counters()
+    n = lib().orc_adaptive_continue(C.byref(field), C.byref(params), int(mode), _dp(last),
+                                    float(mu), float(v), mass, charge, delta, _dp(rows), cap,
+                                    C.byref(nsw), C.byref(st), C.byref(cnt))
+    return rows[:n].copy(), dict(nswitch=nsw.value, status=st.value, naccpt=cnt.naccpt,
+                                 nrejct=cnt.nrejct, nfcn=cnt.nfcn)
+
+
+def particle_isadiabatic(field, params, row7, mass, charge):
+    return bool(lib().orc_particle_isadiabatic(C.byref(field), C.byref(params),
+                                               _dp(np.ascontiguousarray(row7, float)), mass,
+                                               charge))
+
+
+def gc_isadiabatic(field, params, row5, mass, charge, mu):
+    return bool(lib().orc_gc_isadiabatic(C.byref(field), C.byref(params),
+                                         _dp(np.ascontiguousarray(row5, float)), mass, charge,
+                                         mu))
+
+
+def adaptive_to_gc(field, row7, mass, charge):
+    """GuidingCenter.init(Particle): (row5, mu, v), or None without convergence."""
+    row5 = np.zeros(5)
+    mu, v = C.c_double(), C.c_double()
+    rc = lib().orc_adaptive_to_gc(C.byref(field), _dp(np.ascontiguousarray(row7, float)), mass,
+                                  charge, _dp(row5), C.byref(mu), C.byref(v))
+    if rc != 1:
+        return None
+    return row5, mu.value, v.value
+
+
+def adaptive_to_particle(field, row5, mass, charge, mu):
+    """Particle.init(GuidingCenter): row7."""
+    row7 = np.zeros(7)
+    lib().orc_adaptive_to_particle(C.byref(field), _dp(np.ascontiguousarray(row5, float)), mass,
+                                   charge, mu, _dp(row7))
+    return row7
 
 
 def guidingcenter(field, t, r, v, mass, charge, tol=1e-3, maxiter=20):

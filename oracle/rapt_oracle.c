@@ -1177,28 +1177,33 @@ static void put_row8(double *dst, int mode, const double *src, double mu) {
     }
 }
 
-int64_t orc_adaptive_run(const orc_field *f, const orc_params *p, const double pos[3],
-                         const double vel[3], double t0, double mass, double charge,
-                         double delta, double *rows, int64_t cap, int32_t *nswitch,
-                         int32_t *status, orc_counters *cnt) {
-    orc_params pp = *p;
-    pp.check_adiabaticity = 1;
-    double prow[7], grow[5], mu = 0.0, v = 0.0;
-    int64_t n = 0;
-    int mode;
-    *nswitch = 0;
-    *status = ORC_OK;
-    if (pp.gctimestep <= 0.0) { *status = -10; return 0; }
-    particle_init_row(t0, pos, vel, mass, prow);
-    if (particle_isadiabatic(f, &pp, prow, mass, charge)) {
-        int rc = gc_from_particle(f, prow, mass, charge, grow, &mu, &v);
-        if (rc != ORC_OK) { *status = rc; return 0; }
-        mode = 1;
-    } else {
-        mode = 0;
-    }
-    if (n >= cap) { *status = ORC_ERR_CAP; return n; }
-    put_row8(rows + 8 * n++, mode, mode == 0 ? prow : grow, mu);
+int orc_particle_isadiabatic(const orc_field *f, const orc_params *p, const double row7[7],
+                             double mass, double charge) {
+    return particle_isadiabatic(f, p, row7, mass, charge);
+}
+int orc_gc_isadiabatic(const orc_field *f, const orc_params *p, const double row5[5],
+                       double mass, double charge, double mu) {
+    return gc_isadiabatic(f, p, row5, mass, charge, mu);
+}
+int orc_adaptive_to_gc(const orc_field *f, const double row7[7], double mass, double charge,
+                       double row5[5], double *mu, double *v) {
+    return gc_from_particle(f, row7, mass, charge, row5, mu, v);
+}
+void orc_adaptive_to_particle(const orc_field *f, const double row5[5], double mass,
+                              double charge, double mu, double row7[7]) {
+    particle_from_gc(f, row5, mass, charge, mu, row7);
+}
+
+/* The switching loop of Adaptive.advance (rapt/Adaptive.py:187-222) from the
+ * last tracer of trajlist: mode 0 a Particle with last row prow, mode 1 a
+ * GuidingCenter with last row grow, magnetic moment mu and speed v.  The
+ * first segment's rows continue that tracer (no start row); every switch
+ * appends the new tracer's start row.  As written there, t starts at 0 and
+ * becomes the new tracer's absolute tcur after a switch. */
+static int64_t adaptive_loop(const orc_field *f, const orc_params *pp, int mode, double *prow,
+                             double *grow, double mu, double v, double mass, double charge,
+                             double delta, double *rows, int64_t n, int64_t cap,
+                             int32_t *nswitch, int32_t *status, orc_counters *cnt) {
     double t = 0.0;
     double *buf = (double *)malloc(sizeof(double) * 7 * (size_t)(cap > 0 ? cap : 1));
     while (t < delta) {
@@ -1206,22 +1211,22 @@ int64_t orc_adaptive_run(const orc_field *f, const orc_params *p, const double p
         double tc = 0.0;
         int64_t m;
         if (mode == 0) {
-            m = orc_particle_advance(f, &pp, prow, mass, charge, delta - t, buf, cap - n, &tc,
+            m = orc_particle_advance(f, pp, prow, mass, charge, delta - t, buf, cap - n, &tc,
                                      &st, &sw, cnt);
             for (int64_t i = 0; i < m; ++i) put_row8(rows + 8 * (n + i), 0, buf + 7 * i, 0.0);
             n += m;
-            if (m > 0) memcpy(prow, buf + 7 * (m - 1), sizeof prow);
+            if (m > 0) memcpy(prow, buf + 7 * (m - 1), 7 * sizeof(double));
         } else {
             /* GuidingCenter.advance sets up scipy dopri5 with its defaults
              * (rapt/GuidingCenter.py:450) */
-            orc_params pg = pp;
+            orc_params pg = *pp;
             pg.stepper = ORC_STEP_DOPRI5;
             pg.safety = 0.9; pg.dfactor = 0.2; pg.ifactor = 10.0; pg.beta = 0.0;
-            m = orc_gc_advance(f, &pg, ORC_EOM_TCB, grow, mass, charge, mu, v, pp.gctimestep,
+            m = orc_gc_advance(f, &pg, ORC_EOM_TCB, grow, mass, charge, mu, v, pp->gctimestep,
                                delta - t, buf, cap - n, &tc, &st, &sw, cnt);
             for (int64_t i = 0; i < m; ++i) put_row8(rows + 8 * (n + i), 1, buf + 5 * i, mu);
             n += m;
-            if (m > 0) memcpy(grow, buf + 5 * (m - 1), sizeof grow);
+            if (m > 0) memcpy(grow, buf + 5 * (m - 1), 5 * sizeof(double));
         }
         if (st != ORC_OK) { *status = st; break; }
         if (sw) {
@@ -1245,6 +1250,49 @@ int64_t orc_adaptive_run(const orc_field *f, const orc_params *p, const double p
     }
     free(buf);
     return n;
+}
+
+int64_t orc_adaptive_run(const orc_field *f, const orc_params *p, const double pos[3],
+                         const double vel[3], double t0, double mass, double charge,
+                         double delta, double *rows, int64_t cap, int32_t *nswitch,
+                         int32_t *status, orc_counters *cnt) {
+    orc_params pp = *p;
+    pp.check_adiabaticity = 1;
+    double prow[7], grow[5], mu = 0.0, v = 0.0;
+    int64_t n = 0;
+    int mode;
+    *nswitch = 0;
+    *status = ORC_OK;
+    if (pp.gctimestep <= 0.0) { *status = -10; return 0; }
+    /* Adaptive.__init__ (rapt/Adaptive.py:70-104) */
+    particle_init_row(t0, pos, vel, mass, prow);
+    if (particle_isadiabatic(f, &pp, prow, mass, charge)) {
+        int rc = gc_from_particle(f, prow, mass, charge, grow, &mu, &v);
+        if (rc != ORC_OK) { *status = rc; return 0; }
+        mode = 1;
+    } else {
+        mode = 0;
+    }
+    if (n >= cap) { *status = ORC_ERR_CAP; return n; }
+    put_row8(rows + 8 * n++, mode, mode == 0 ? prow : grow, mu);
+    return adaptive_loop(f, &pp, mode, prow, grow, mu, v, mass, charge, delta, rows, n, cap,
+                         nswitch, status, cnt);
+}
+
+int64_t orc_adaptive_continue(const orc_field *f, const orc_params *p, int mode,
+                              const double *last, double mu, double v, double mass,
+                              double charge, double delta, double *rows, int64_t cap,
+                              int32_t *nswitch, int32_t *status, orc_counters *cnt) {
+    orc_params pp = *p;
+    pp.check_adiabaticity = 1;
+    double prow[7] = {0}, grow[5] = {0};
+    *nswitch = 0;
+    *status = ORC_OK;
+    if (pp.gctimestep <= 0.0) { *status = -10; return 0; }
+    if (mode == 0) memcpy(prow, last, sizeof prow);
+    else memcpy(grow, last, sizeof grow);
+    return adaptive_loop(f, &pp, mode, prow, grow, mu, v, mass, charge, delta, rows, 0, cap,
+                         nswitch, status, cnt);
 }
 
 /* ======================================================================

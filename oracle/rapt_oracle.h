@@ -155,6 +155,30 @@ int64_t orc_adaptive_run(const orc_field *f, const orc_params *p, const double p
                          const double vel[3], double t0, double mass, double charge,
                          double delta, double *rows, int64_t cap, int32_t *nswitch,
                          int32_t *status, orc_counters *cnt);
+/* A further Adaptive.advance(delta) from the last tracer of trajlist: mode 0 a
+ * Particle whose last row is last[7], mode 1 a GuidingCenter with last row
+ * last[5], moment mu and speed v.  Rows as orc_adaptive_run, without a start
+ * row for the continued tracer. */
+int64_t orc_adaptive_continue(const orc_field *f, const orc_params *p, int mode,
+                              const double *last, double mu, double v, double mass,
+                              double charge, double delta, double *rows, int64_t cap,
+                              int32_t *nswitch, int32_t *status, orc_counters *cnt);
+/* Particle.isadiabatic (rapt/Particle.py:345-384) of a Particle whose last row
+ * is row7, and GuidingCenter.isadiabatic (rapt/GuidingCenter.py:287-327) of a
+ * guiding centre with last row row5 and moment mu: 1 adiabatic, 0 not. */
+int orc_particle_isadiabatic(const orc_field *f, const orc_params *p, const double row7[7],
+                             double mass, double charge);
+int orc_gc_isadiabatic(const orc_field *f, const orc_params *p, const double row5[5],
+                       double mass, double charge, double mu);
+/* The conversions of a switch: GuidingCenter.init(Particle)
+ * (rapt/GuidingCenter.py:168-186, then __init__ :123-133) gives row5, mu and
+ * the speed v (returns 1, or the fixed-point iteration's error); and
+ * Particle.init(GuidingCenter) (rapt/Particle.py:149-164, GCtoFP at
+ * gyrophase 0 and the new Particle's t = 0) gives row7. */
+int orc_adaptive_to_gc(const orc_field *f, const double row7[7], double mass, double charge,
+                       double row5[5], double *mu, double *v);
+void orc_adaptive_to_particle(const orc_field *f, const double row5[5], double mass,
+                              double charge, double mu, double row7[7]);
 
 /* batch drivers with OpenMP over particles (CPU baseline).  SoA-free: state is
  * an array of per-particle records. */

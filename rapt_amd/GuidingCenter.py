@@ -17,6 +17,8 @@ from . import utils as ru
 from .constants import c
 
 _EOMS = ("TaoChanBrizardEOM", "BrizardChanEOM", "NorthropTellerEOM")
+# field models whose half-bounce path the device computes (rapt_halfbounce)
+_BOUNCE_MODELS = (1, 2, 5, 8, 9)
 
 
 class GuidingCenter:
@@ -131,18 +133,47 @@ class GuidingCenter:
         return self.bounceperiod() / P["bounceresolution"]
 
     @staticmethod
-    def _advance_many(gs, delta, eom="TaoChanBrizardEOM", **kw):
-        from . import NonAdiabatic, batch
-        from . import params as P
+    def _advance_many(gs, delta, eom="TaoChanBrizardEOM", device_dt=False, **kw):
+        """Advance GuidingCenters that share one field.  Those that check
+        adiabaticity run in a launch of their own, so that a switch point stops
+        only them.  device_dt: output steps bounceperiod()/bounceresolution from
+        the device half-bounce kernel (within the reference's 1e-4 quadrature
+        tolerance) instead of one host quadrature per tracer."""
+        from . import NonAdiabatic
         if eom not in _EOMS:
             raise ValueError("unknown eom %r" % (eom,))
+        raise_na = False
+        for check in (True, False):
+            grp = [g for g in gs if bool(g.check_adiabaticity) == check]
+            if grp:
+                raise_na |= GuidingCenter._advance_group(grp, delta, eom, check, device_dt, **kw)
+        if raise_na:
+            raise NonAdiabatic
+
+    @staticmethod
+    def _batch_dt(gs, device_dt):
+        from . import batch
+        from . import params as P
+        if P["GCtimestep"] != 0:
+            return np.full(len(gs), float(P["GCtimestep"]))
+        if device_dt and gs[0].field._model in _BOUNCE_MODELS:
+            last = np.stack([g.trajectory[-1] for g in gs])
+            dt, _ = batch.gc_bounce_dt(gs[0].field, last, np.array([g.mass for g in gs], float),
+                                       np.array([g.mu for g in gs], float), params=P)
+            return dt
+        return np.array([g._dt() for g in gs], dtype=float)
+
+    @staticmethod
+    def _advance_group(gs, delta, eom, check, device_dt, **kw):
+        from . import batch
+        from . import params as P
         last = np.stack([g.trajectory[-1] for g in gs])
-        dt = np.array([g._dt() for g in gs], dtype=float)
+        dt = GuidingCenter._batch_dt(gs, device_dt)
         res = batch.advance_gcs(gs[0].field, last, np.array([g.mass for g in gs], float),
                                 np.array([g.charge for g in gs], float),
                                 np.array([g.mu for g in gs], float),
                                 np.array([g.v for g in gs], float), dt, delta, params=P,
-                                eom=eom, check=any(g.check_adiabaticity for g in gs), **kw)
+                                eom=eom, check=check, **kw)
         raise_na = False
         for i, g in enumerate(gs):
             rows = res.rows[i]
@@ -157,9 +188,8 @@ class GuidingCenter:
                 import warnings
                 from ._native import STATUS_TEXT
                 warnings.warn("GuidingCenter.advance: %s" % STATUS_TEXT.get(int(st), st),
-                              stacklevel=3)
-        if raise_na:
-            raise NonAdiabatic
+                              stacklevel=4)
+        return raise_na
 
     # ---- accessors (rapt/GuidingCenter.py:460-624) ---------------------------------------
     def gett(self):

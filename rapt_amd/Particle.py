@@ -103,12 +103,25 @@ class Particle:
 
     @staticmethod
     def _advance_many(ps, delta, **kw):
-        from . import Adiabatic, batch
+        """Advance Particles that share one field.  Those that check
+        adiabaticity run in a launch of their own, so that a switch point stops
+        only them and the others reach t0 + delta as the reference traces them."""
+        from . import Adiabatic
+        raise_adiabatic = False
+        for check in (True, False):
+            grp = [p for p in ps if bool(p.check_adiabaticity) == check]
+            if grp:
+                raise_adiabatic |= Particle._advance_group(grp, delta, check, **kw)
+        if raise_adiabatic:
+            raise Adiabatic
+
+    @staticmethod
+    def _advance_group(ps, delta, check, **kw):
+        from . import batch
         from . import params as P
         last = np.stack([p.trajectory[-1] for p in ps])
         mass = np.array([p.mass for p in ps], dtype=float)
         charge = np.array([p.charge for p in ps], dtype=float)
-        check = any(p.check_adiabaticity for p in ps)
         res = batch.advance_particles(ps[0].field, last, mass, charge, delta, params=P,
                                       check=check, **kw)
         raise_adiabatic = False
@@ -125,9 +138,8 @@ class Particle:
                 import warnings
                 from ._native import STATUS_TEXT
                 warnings.warn("Particle.advance: %s" % STATUS_TEXT.get(int(st), st),
-                              stacklevel=3)
-        if raise_adiabatic:
-            raise Adiabatic
+                              stacklevel=4)
+        return raise_adiabatic
 
     # ---- persistence --------------------------------------------------------------
     def save(self, filename):

@@ -235,9 +235,48 @@ def advance_adaptive(field, pos, vel, t0, mass, charge, delta, params=None, row_
     return _run(field, N.MODE_ADAPTIVE, S, flags, row_cap, desc, device)
 
 
-def advance(tracers, delta, **kw):
+def continue_adaptive(field, mode, last, mass, charge, mu, v, delta, params=None,
+                      row_cap=None, lanes=0, device=0, **kw):
+    """A further Adaptive.advance(delta) of a batch from each tracer's last
+    segment (rapt/Adaptive.py:187-222): mode[i] 0 continues a Particle whose
+    last row is last[i] = (t, x, y, z, px, py, pz), 1 a GuidingCenter with
+    last row (t, x, y, z, ppar, ...), moment mu[i] and speed v[i].  Rows as
+    advance_adaptive; the continued segment gets no start row."""
+    P = _params(params)
+    last = np.atleast_2d(np.asarray(last, dtype=np.float64))
+    n = last.shape[0]
+    mode = np.broadcast_to(np.asarray(mode, np.int32), (n,))
+    S = np.zeros((N.NSLOT, n))
+    S[N.S_T] = last[:, 0]
+    gc = mode == 1
+    S[N.S_Y0:N.S_Y0 + 6, ~gc] = last[~gc, 1:7].T
+    S[N.S_Y0:N.S_Y0 + 4, gc] = last[gc, 1:5].T
+    S[N.S_MASS] = np.broadcast_to(mass, (n,))
+    S[N.S_CHARGE] = np.broadcast_to(charge, (n,))
+    S[N.S_MU] = np.where(gc, np.broadcast_to(mu, (n,)), 0.0)
+    S[N.S_V] = np.where(gc, np.broadcast_to(v, (n,)), 0.0)
+    flags = np.where(gc, N.FLAG_GCMODE, 0).astype(np.int32)
+    if row_cap is None:
+        gcdt = float(P.get("GCtimestep", 0.0)) or 1.0
+        dtp = np.full(n, gcdt)
+        if np.any(~gc):
+            dtp[~gc] = particle_dt(field, last[~gc, :7], np.broadcast_to(mass, (n,))[~gc],
+                                   np.broadcast_to(charge, (n,))[~gc], P["cyclotronresolution"],
+                                   device)
+        row_cap = _rows_needed(delta, np.minimum(dtp, gcdt)) + 8
+    desc = N.make_desc(N.MODE_ADAPTIVE, P, delta, lanes=lanes, **kw)
+    return _run(field, N.MODE_ADAPTIVE, S, flags, row_cap, desc, device)
+
+
+def advance(tracers, delta, device_dt=True, **kw):
     """Advance a list of Particle / GuidingCenter objects together (grouped by
-    type and field); equivalent to calling ``t.advance(delta)`` on each."""
+    type and field; tracers that check adiabaticity launch apart from those
+    that do not); equivalent to calling ``t.advance(delta)`` on each.
+
+    device_dt: a group of guiding centres takes its output steps
+    bounceperiod()/bounceresolution from the device half-bounce kernel (within
+    the reference's 1e-4 quadrature tolerance) rather than from one host
+    quadrature per tracer, which is what a single ``advance`` uses."""
     from .GuidingCenter import GuidingCenter
     from .Particle import Particle
     groups = {}
@@ -247,7 +286,7 @@ def advance(tracers, delta, **kw):
         if tp is Particle:
             Particle._advance_many(ts, delta, **kw)
         elif tp is GuidingCenter:
-            GuidingCenter._advance_many(ts, delta, **kw)
+            GuidingCenter._advance_many(ts, delta, device_dt=device_dt, **kw)
         else:
             for t in ts:
                 t.advance(delta)

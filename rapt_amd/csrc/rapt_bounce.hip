@@ -25,6 +25,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <cmath>
+
 #include <vector>
 
 #include "../../include/rapt_hip.h"
@@ -222,6 +224,16 @@ __device__ __forceinline__ int32_t line_integrals(const BounceArgs& A, int64_t l
   const FieldC& F = A.F;
   const int32_t base = A.cap;  // array index of the start point
   const double ds = 1.0 / curvature<M>(F, t, x0, y0, z0) / A.res;
+  // A chunk length that is not finite and positive (a NaN start point or
+  // mirror field, a zero or infinite curvature such as |B| = 0 or a straight
+  // line) would let a chunk end without adding a point, and the trace below
+  // would never reach its mirror field.
+  if (!(ds > 0.0 && ds < __builtin_inf()) || !(Bm > 0.0 && Bm < __builtin_inf())) {
+    if (sb_out) *sb_out = __builtin_nan("");
+    if (eye_out) *eye_out = __builtin_nan("");
+    if (lat_out) *lat_out = __builtin_nan("");
+    return BST_NAN;
+  }
   A.S[(int64_t)base * A.lanes + lane] = 0.0;
   A.Bv[(int64_t)base * A.lanes + lane] = magB<M>(F, t, x0, y0, z0);
   if (A.Zs) {
@@ -237,9 +249,11 @@ __device__ __forceinline__ int32_t line_integrals(const BounceArgs& A, int64_t l
     int32_t* np = side == 0 ? &nf : &nb;
     double Y[4] = {0.0, x0, y0, z0};
     for (;;) {
+      const int32_t before = *np;
       const int rc = line_chunk<M>(A, lane, t, sign, tol, ds, Y, np, base, dir);
       if (rc < 0) { st = BST_OVERFLOW; break; }
       if (rc == 0) break;
+      if (*np == before) { st = BST_NAN; break; }  // no progress: cannot reach Bm
       if (magB<M>(F, t, Y[1], Y[2], Y[3]) > Bm) break;
     }
   }
@@ -628,11 +642,21 @@ static void launch_lab(rapt_ctx* c, const BounceArgs& A, const LabelArgs& L) {
                      c->stream, A, L);
 }
 
+// True when every one of the n values is finite (a NaN or infinite start point
+// or mirror field has no field line to trace).
+static bool all_finite(const double* v, int64_t n) {
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
 // Shared host side: device copies of the inputs, the per-lane trace scratch,
 // one launch, results back.  in_w doubles in per query, out_w doubles out.
 static int run_bounce(rapt_ctx* c, int64_t n, const double* in, int in_w, const double* Bm,
                       double res, bool positions, const LabelArgs* lab, double* out,
                       int out_w, int32_t* status, int32_t what = 0) {
+  if (!all_finite(in, (int64_t)in_w * n) || (Bm && !all_finite(Bm, n)))
+    return set_err(c, RAPT_E_ARG, "non-finite start point or mirror field");
   HIP_OK(c, hipSetDevice(c->device));
   BounceArgs A;
   memset(&A, 0, sizeof A);
@@ -757,6 +781,9 @@ extern "C" int rapt_bounce_center_advance(rapt_ctx* c, int64_t n, double* tpos, 
       !(atol > 0.0))
     return set_err(c, RAPT_E_ARG, "bad BounceCenter parameters");
   if (n == 0) return RAPT_OK;
+  if (!all_finite(tpos, 4 * n) || !all_finite(Bm, n) || !all_finite(coef, n) ||
+      !all_finite(tcur, n))
+    return set_err(c, RAPT_E_ARG, "non-finite BounceCenter state, mirror field or coefficient");
   int64_t total = 0;
   std::vector<int64_t> off((size_t)n);
   for (int64_t i = 0; i < n; ++i) {

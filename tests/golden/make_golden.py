@@ -262,6 +262,138 @@ def make_utils():
     np.savez_compressed(os.path.join(HERE, "utils.npz"), **out)
 
 
+def make_switching():
+    """Adiabaticity decisions of the reference's own tracer objects, with the
+    quantities they compare: GuidingCenter.isadiabatic (cycrad, the Q5 cycper
+    with its unnormalised pp**2, lengthscale, timescale) in a static field
+    (DoubleDipole) and a time-dependent one (VarEarthDipole), and
+    Particle.isadiabatic in the time-dependent field (cycper / timescale
+    against epst), each under four (epss, epst) settings."""
+    out = {}
+    n = 24
+    settings = [(5e-2, 5e-2), (2e-3, 5e-2), (5e-2, 2e-2), (5e-2, 0.5)]
+    out["settings"] = np.array(settings)
+    cases = (("gc_dd", rf.DoubleDipole(), 31), ("gc_var", rf.VarEarthDipole(amp=0.2, period=7.0), 32))
+    for name, f, seed in cases:
+        L = W.uniform(seed, 0, n, 2.5, 9.0)
+        ph = W.uniform(seed, 1, n, 0.0, 2 * np.pi)
+        lat = W.uniform(seed, 2, n, -0.5, 0.5)
+        K = 10 ** W.uniform(seed, 3, n, 3.0, 7.5)
+        pa = W.uniform(seed, 4, n, 5.0, 89.0)
+        t0 = W.uniform(seed, 5, n, 0.0, 7.0)
+        proton = np.arange(n) % 2 == 0
+        rows, inp, dec = [], [], []
+        for i in range(n):
+            m, q = (rapt.m_pr, rapt.e) if proton[i] else (rapt.m_el, -rapt.e)
+            r = L[i] * Re * np.cos(lat[i]) ** 2
+            pos = [r * np.cos(lat[i]) * np.cos(ph[i]), r * np.cos(lat[i]) * np.sin(ph[i]),
+                   r * np.sin(lat[i])]
+            v = ru.speedfromKE(K[i], m)
+            g = rapt.GuidingCenter(pos, float(v), float(pa[i]), None, float(t0[i]), m, q, f)
+            tp = g.trajectory[-1, :4]
+            ts = f.timescale(tp)
+            dd = []
+            for es, et in settings:
+                reset_params(epss=es, epst=et)
+                dd.append(bool(g.isadiabatic()))
+            reset_params()
+            rows.append(g.trajectory[-1].copy())
+            inp.append([m, q, g.mu])
+            dec.append(dd + [g.cycrad(), g.cycper(), f.lengthscale(tp),
+                             np.nan if ts is None else ts])
+        out[name + "/row"] = np.array(rows)
+        out[name + "/mqmu"] = np.array(inp)
+        out[name + "/dec"] = np.array(dec, dtype=float)
+    # relativistic electrons at small pitch angles, where cycper's
+    # unnormalised pp**2 (rapt/GuidingCenter.py:535) makes its Lorentz factor
+    # the perpendicular one, and epst thresholds across their ratios
+    f = rf.VarEarthDipole(amp=0.2, period=7.0)
+    seed = 34
+    set_e = [(5e-2, e) for e in (1e-6, 3e-6, 1e-5, 3e-5, 1e-4)]
+    out["settings_e"] = np.array(set_e)
+    L = W.uniform(seed, 0, n, 2.5, 7.0)
+    ph = W.uniform(seed, 1, n, 0.0, 2 * np.pi)
+    K = 10 ** W.uniform(seed, 3, n, 6.0, 7.5)
+    pa = W.uniform(seed, 4, n, 5.0, 45.0)
+    t0 = W.uniform(seed, 5, n, 0.0, 7.0)
+    rows, inp, dec = [], [], []
+    for i in range(n):
+        pos = [L[i] * Re * np.cos(ph[i]), L[i] * Re * np.sin(ph[i]), 0.0]
+        v = ru.speedfromKE(K[i], rapt.m_el)
+        g = rapt.GuidingCenter(pos, float(v), float(pa[i]), None, float(t0[i]), rapt.m_el,
+                               -rapt.e, f)
+        dd = []
+        for es, et in set_e:
+            reset_params(epss=es, epst=et)
+            dd.append(bool(g.isadiabatic()))
+        reset_params()
+        rows.append(g.trajectory[-1].copy())
+        inp.append([rapt.m_el, -rapt.e, g.mu])
+        dec.append(dd + [g.cycper()])
+    out["gc_var_e/row"] = np.array(rows)
+    out["gc_var_e/mqmu"] = np.array(inp)
+    out["gc_var_e/dec"] = np.array(dec, dtype=float)
+    f = rf.VarEarthDipole(amp=0.2, period=7.0)
+    seed = 33
+    L = W.uniform(seed, 0, n, 2.5, 9.0)
+    ph = W.uniform(seed, 1, n, 0.0, 2 * np.pi)
+    K = 10 ** W.uniform(seed, 3, n, 3.0, 7.5)
+    th = np.arccos(W.uniform(seed, 4, n, -1.0, 1.0))
+    pv = W.uniform(seed, 5, n, 0.0, 2 * np.pi)
+    t0 = W.uniform(seed, 6, n, 0.0, 7.0)
+    rows, inp, dec = [], [], []
+    for i in range(n):
+        m, q = (rapt.m_pr, rapt.e) if i % 2 == 0 else (rapt.m_el, -rapt.e)
+        v = ru.speedfromKE(K[i], m)
+        pos = [L[i] * Re * np.cos(ph[i]), L[i] * Re * np.sin(ph[i]), 0.3 * Re]
+        vel = [v * np.sin(th[i]) * np.cos(pv[i]), v * np.sin(th[i]) * np.sin(pv[i]),
+               v * np.cos(th[i])]
+        p = rapt.Particle(pos, vel, float(t0[i]), m, q, f)
+        tp = p.trajectory[-1, :4]
+        dd = []
+        for es, et in settings:
+            reset_params(epss=es, epst=et)
+            dd.append(bool(p.isadiabatic()))
+        reset_params()
+        rows.append(p.trajectory[-1].copy())
+        inp.append([m, q])
+        dec.append(dd + [p.cycrad(), p.cycper(), f.lengthscale(tp), f.timescale(tp)])
+    out["p_var/row"] = np.array(rows)
+    out["p_var/mq"] = np.array(inp)
+    out["p_var/dec"] = np.array(dec, dtype=float)
+    np.savez_compressed(os.path.join(HERE, "switching.npz"), **out)
+
+
+def make_accessors():
+    """The reference's accessors on its own golden trajectories: Particle
+    (rapt/Particle.py:386-482) on the six advance(10) protons, GuidingCenter
+    (rapt/GuidingCenter.py:460-591) on the six advance(3) electrons.  Particle
+    getv and getB, and GuidingCenter getp and getv, are not recorded: they
+    return NaN, None, a wrong relativistic value or raise there."""
+    out = {}
+    P = np.load(os.path.join(HERE, "particles.npz"))
+    G = np.load(os.path.join(HERE, "gcs.npz"))
+    off = np.r_[0, np.cumsum(P["c1/len"])]
+    for k in range(3):
+        p = rapt.Particle()
+        p.mass, p.charge, p.field = rapt.m_pr, rapt.e, rf.EarthDipole()
+        p.trajectory = P["c1/traj"][off[k]:off[k + 1]].copy()
+        for nm in ("gett", "getx", "gety", "getz", "getpx", "getpy", "getpz", "getp",
+                   "getgamma", "getvx", "getvy", "getvz", "getr", "gettheta", "getphi",
+                   "getke", "mu", "guidingcenter"):
+            out["p%d/%s" % (k, nm)] = np.asarray(getattr(p, nm)(), dtype=float)
+    off = np.r_[0, np.cumsum(G["c2/len"])]
+    for k in range(3):
+        g = rapt.GuidingCenter()
+        g.mass, g.charge, g.field = rapt.m_el, -rapt.e, rf.DoubleDipole()
+        g.mu = float(G["c2/mu"][k])
+        g.trajectory = G["c2/traj"][off[k]:off[k + 1]].copy()
+        for nm in ("gett", "getx", "gety", "getz", "getpp", "getr", "gettheta", "getphi",
+                   "getB", "getgamma", "getBm", "getke"):
+            out["g%d/%s" % (k, nm)] = np.asarray(getattr(g, nm)(), dtype=float)
+    np.savez_compressed(os.path.join(HERE, "accessors.npz"), **out)
+
+
 def make_rkf():
     out = {}
     fn = lambda x, t: np.array([x[1], -x[0]])
@@ -294,8 +426,14 @@ def make_bounce():
     np.savez_compressed(os.path.join(HERE, "bounce.npz"), **out)
 
 
+ALL = (make_fields, make_particles, make_gcs, make_utils, make_rkf, make_bounce,
+       make_switching, make_accessors)
+
 if __name__ == "__main__":
-    for fn in (make_fields, make_particles, make_gcs, make_utils, make_rkf, make_bounce):
+    # with arguments, only the named fixtures (e.g. make_switching)
+    for fn in ALL:
+        if len(sys.argv) > 1 and fn.__name__ not in sys.argv[1:]:
+            continue
         t = time.time()
         fn()
         print(fn.__name__, "%.1f s" % (time.time() - t), flush=True)
