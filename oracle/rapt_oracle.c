@@ -699,7 +699,7 @@ int orc_rkf(orc_rhs_fn fn, void *ctx, int n, double a, double b, const double *x
         h = h * dmin(dmax(0.84 * q, 0.1), 4.0);
         if (h > hmax) {
             h = hmax;
-        } else if (h < hmin) {
+        } else if (h < hmin || !(h > 0.0)) {
             rc = ORC_ERR_HMIN;
             break;
         }
@@ -1015,7 +1015,7 @@ static int rkf_interval(orc_rhs_fn fn, void *ctx, int n, double *x, double *y, d
         h = h * dmin(dmax(0.84 * pow(tol / r, 0.25), 0.1), 4.0);
         if (h > hmax) {
             h = hmax;
-        } else if (h < hmin) {
+        } else if (h < hmin || !(h > 0.0)) {
             rc = ORC_ERR_HMIN;
             break;
         }

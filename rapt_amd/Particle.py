@@ -214,8 +214,10 @@ class Particle:
 
     def getke(self):
         g = self.getgamma()
-        ke_nr = 0.5 * np.sum(self.trajectory[:, 4:7] ** 2, axis=1) / self.mass
-        return np.where(g - 1 < 1e-6, ke_nr, (g - 1) * self.mass * c * c)
+        t = self.trajectory
+        ke_nr = 0.5 * (t[:, 4] ** 2 + t[:, 5] ** 2 + t[:, 6] ** 2) / self.mass
+        mc2 = self.mass * c * c
+        return np.where(g - 1 < 1e-6, ke_nr, (g - 1) * mc2)
 
     def getB(self):
         from . import _native as N

@@ -240,11 +240,16 @@ def continue_adaptive(field, mode, last, mass, charge, mu, v, delta, params=None
     """A further Adaptive.advance(delta) of a batch from each tracer's last
     segment (rapt/Adaptive.py:187-222): mode[i] 0 continues a Particle whose
     last row is last[i] = (t, x, y, z, px, py, pz), 1 a GuidingCenter with
-    last row (t, x, y, z, ppar, ...), moment mu[i] and speed v[i].  Rows as
+    last row (t, x, y, z, ppar), moment mu[i] and speed v[i] (last: an array
+    of rows of one width, or a list of rows of either width).  Rows as
     advance_adaptive; the continued segment gets no start row."""
     P = _params(params)
-    last = np.atleast_2d(np.asarray(last, dtype=np.float64))
-    n = last.shape[0]
+    rows = [np.asarray(r, dtype=np.float64).ravel() for r in np.atleast_1d(last)] \
+        if isinstance(last, (list, tuple)) else list(np.atleast_2d(np.asarray(last, float)))
+    n = len(rows)
+    last = np.zeros((n, 7))
+    for i, r in enumerate(rows):
+        last[i, :min(7, r.size)] = r[:7]
     mode = np.broadcast_to(np.asarray(mode, np.int32), (n,))
     S = np.zeros((N.NSLOT, n))
     S[N.S_T] = last[:, 0]

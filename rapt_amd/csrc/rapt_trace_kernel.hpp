@@ -351,7 +351,9 @@ void k_trace(KArgs A) {
         h = h * fmin(fmax(0.84 * pow(A.rkf_tol / err, 0.25), 0.1), 4.0);
         if (h > A.rkf_hmax) {
           h = A.rkf_hmax;
-        } else if (h < A.rkf_hmin) {
+        } else if (h < A.rkf_hmin || !(h > 0.0)) {
+          // with hmin = 0, a non-finite error shrinks h to zero, where rkf
+          // would repeat a zero step for ever
           fail = RAPT_ST_HMIN;
         }
         if (!fail && !(x < xend)) call_done = true;

@@ -547,8 +547,11 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
     return set_err(c, RAPT_E_ARG, "ADAPTIVE needs params['GCtimestep'] > 0");
   if (d->mode == RAPT_MODE_PARTICLE && !(d->cyclotronresolution > 0.0))
     return set_err(c, RAPT_E_ARG, "cyclotronresolution must be > 0");
-  if (d->stepper == RAPT_STEP_RKF45 && !(d->rkf_hmax > 0.0 && d->rkf_hmin > 0.0))
-    return set_err(c, RAPT_E_ARG, "RKF45 needs rkf_hmax, rkf_hmin > 0");
+  // rkf (rapt/rkf.py:13-143) takes any hmin; 0 never stops it, which the
+  // fixed-step mode (tol = inf) needs: the step that ends an output interval
+  // can be a rounding remainder far below any positive hmin
+  if (d->stepper == RAPT_STEP_RKF45 && !(d->rkf_hmax > 0.0 && d->rkf_hmin >= 0.0))
+    return set_err(c, RAPT_E_ARG, "RKF45 needs rkf_hmax > 0 and rkf_hmin >= 0");
   if (d->block > 0 && (d->block % 64 != 0 || d->block > 256))
     return set_err(c, RAPT_E_ARG, "block must be 64..256, x64");
   HIP_OK(c, hipSetDevice(c->device));

@@ -1,0 +1,471 @@
+"""GPU tests of the object API and of the paths the parity suite does not
+reach: Adaptive switching against the oracle and the closed-form current
+sheet, repeated Adaptive.advance, mixed check_adiabaticity batches, the
+failure statuses, the RKF45 tracer stepper, save/load mid-run, the
+device-backed accessors against the reference's outputs, and the
+mixed-precision configuration (EarthDipole + (0, 0, -10 nT), protons and
+electrons, guiding centres for 30 s plus a Particle sub-batch).
+
+Tolerances (per test): adaptive positions at the output times 1e-7 relative;
+fixed step counts 1e-10; first switch times equal to 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+import rapt_amd as rapt
+import rapt_amd.fields as F
+from rapt_amd import _native as N
+from rapt_amd import batch
+from rapt_amd import constants as K
+from rapt_amd import workloads as W
+
+from _sheet import segments, sheet_ratio
+
+pytestmark = pytest.mark.gpu
+
+C = float(K.c)
+
+
+def P(**kw):
+    d = dict(rapt.params)
+    d.update(kw)
+    return d
+
+
+def pos_err(a, b):
+    d = np.linalg.norm(a[:, 1:4] - b[:, 1:4], axis=1)
+    return float(np.max(d / np.linalg.norm(b[:, 1:4], axis=1))) if len(a) else 0.0
+
+
+def _first_switch(rows):
+    k = np.nonzero(np.diff(rows[:, 0]) != 0)[0]
+    return int(k[0]) + 1 if k.size else None
+
+
+def test_adaptive_sheet_first_switch_matches_oracle(oracle):
+    """Adaptive in the current sheet, 48 protons, 40 s: for every particle the
+    same initial mode, the same rows up to and including the first switch
+    (positions 1e-7), and the first switch at the same row and time
+    (rtol 1e-12), or no switch on either side."""
+    n = 48
+    wl = W.current_sheet_protons(n=n, seed=11)
+    f = F.CurrentSheet()
+    res = batch.advance_adaptive(f, wl["pos"], wl["vel"], 0.0, K.m_pr, K.e, 40.0,
+                                 params=P(GCtimestep=0.5))
+    fo = oracle.field_from_desc(f.descriptor())
+    po = oracle.particle_params(gctimestep=0.5)
+    switched = 0
+    for i in range(n):
+        ro, info = oracle.adaptive_run(fo, po, wl["pos"][i], wl["vel"][i], 0.0, K.m_pr, K.e,
+                                       40.0)
+        got = res.rows[i]
+        assert got[0, 0] == ro[0, 0], i
+        kg, ko = _first_switch(got), _first_switch(ro)
+        assert kg == ko, (i, kg, ko)
+        k = len(ro) if ko is None else ko + 1
+        k = min(k, len(got))
+        np.testing.assert_allclose(got[:k, 1], ro[:k, 1], rtol=1e-12, atol=0)
+        d = np.linalg.norm(got[:k, 2:5] - ro[:k, 2:5], axis=1) / np.linalg.norm(ro[:k, 2:5],
+                                                                                 axis=1)
+        assert np.max(d) < 1e-7, (i, np.max(d))
+        switched += ko is not None
+    assert switched >= 8
+
+
+def test_adaptive_sheet_switch_rows_straddle_epss_on_device():
+    """The device's own rows: every row at which Adaptive tests adiabaticity is
+    on the side of epss its segment requires, and each switch row on the
+    other side, with rho/L in closed form (independent of any oracle)."""
+    n = 256
+    wl = W.current_sheet_protons(n=n, seed=12)
+    res = batch.advance_adaptive(F.CurrentSheet(), wl["pos"], wl["vel"], 0.0, K.m_pr, K.e, 60.0,
+                                 params=P(GCtimestep=0.5))
+    assert np.all(res.status == N.ST_DONE)
+    epss, tiny = 5e-2, 1e-6
+    nsw = 0
+    for i in range(n):
+        rows = res.rows[i]
+        g = 1 / np.sqrt(1 - wl["vel"][i] @ wl["vel"][i] / C ** 2)
+        r0 = sheet_ratio(np.r_[0.0, wl["pos"][i], K.m_pr * g * wl["vel"][i]], 0, 0.0,
+                         K.m_pr, K.e)
+        assert (r0 < epss * (1 + tiny)) if rows[0, 0] == 1 else (r0 >= epss * (1 - tiny))
+        segs = segments(rows)
+        for j, seg in enumerate(segs):
+            mode = int(seg[0, 0])
+            mu = seg[0, 6] if mode == 1 else 0.0
+            r = np.array([sheet_ratio(row[1:], mode, mu, K.m_pr, K.e) for row in seg])[1:]
+            switched = j < len(segs) - 1
+            body = r[:-1] if switched else r
+            if mode == 0:
+                assert np.all(body >= epss * (1 - tiny))
+                assert not switched or r[-1] < epss * (1 + tiny)
+            else:
+                assert np.all(body < epss * (1 + tiny))
+                assert not switched or r[-1] >= epss * (1 - tiny)
+            nsw += switched
+        assert 60.0 <= rows[-1, 1] < 60.0 + 6.6
+    assert nsw >= 100
+
+
+def test_adaptive_advance_twice_vs_oracle(oracle):
+    """Adaptive.advance(30) then advance(20) on the object: the second call
+    continues the last tracer of trajlist (no new start row), matching the
+    oracle's continuation from the same state up to its first switch, and
+    ends after a switch whose tcur is past this call's delta."""
+    f = F.CurrentSheet()
+    fo = oracle.field_from_desc(f.descriptor())
+    prm = oracle.particle_params(gctimestep=0.5)
+    wl = W.current_sheet_protons(n=12, seed=7)
+    old = dict(rapt.params)
+    rapt.params["GCtimestep"] = 0.5
+    modes = set()
+    try:
+        for i in range(12):
+            a = rapt.Adaptive(wl["pos"][i], wl["vel"][i], 0.0, K.m_pr, K.e, f)
+            a.advance(30.0)
+            cur = a.trajlist[-1]
+            n_before = len(cur.trajectory)
+            last = cur.trajectory[-1].copy()
+            ntr = len(a.trajlist)
+            gc = isinstance(cur, rapt.GuidingCenter)
+            mu = cur.mu if gc else 0.0
+            v = a._gc_speed(last, mu) if gc else 0.0
+            a.advance(20.0)
+            ro, info = oracle.adaptive_continue(fo, prm, int(gc), last, mu, v, K.m_pr, K.e, 20.0)
+            assert info["status"] == 1
+            # rows of the continued tracer, then those of the new ones
+            cont = cur.trajectory[n_before:]
+            k = _first_switch(ro)
+            k = len(ro) if k is None else k
+            assert len(cont) == k, (i, len(cont), k)
+            w = 5 if gc else 7
+            d = np.linalg.norm(cont[:, 1:4] - ro[:k, 2:5], axis=1) / \
+                np.linalg.norm(ro[:k, 2:5], axis=1)
+            assert np.max(d, initial=0.0) < 1e-7
+            np.testing.assert_allclose(cont[:, 0], ro[:k, 1], rtol=1e-12)
+            assert len(a.trajlist) == ntr + (1 if k < len(ro) else 0)
+            if k < len(ro):
+                assert len(ro) == k + 1 and len(a.trajlist[-1].trajectory) == 1
+                np.testing.assert_allclose(a.trajlist[-1].trajectory[0, 0], ro[k, 1], rtol=1e-12)
+            assert cur.trajectory.shape[1] == w
+            modes.add((gc, k < len(ro)))
+    finally:
+        rapt.params.clear()
+        rapt.params.update(old)
+    assert len(modes) >= 2
+
+
+def test_adaptive_setke_setpa_then_advance():
+    f = F.CurrentSheet()
+    wl = W.current_sheet_protons(n=4, seed=3)
+    old = dict(rapt.params)
+    rapt.params["GCtimestep"] = 0.5
+    try:
+        for i in range(4):
+            a = rapt.Adaptive(wl["pos"][i], wl["vel"][i], 0.0, K.m_pr, K.e, f)
+            a.advance(5.0)
+            a.setke(2e3)
+            assert len(a.trajlist) == 1 and len(a.gett()) == 1
+            a.setpa(60.0)
+            a.advance(5.0)
+            t = a.gett()
+            assert t[0] == 0.0 and t[-1] >= 5.0 and np.all(np.diff(t) >= 0)
+            assert np.all(np.isfinite(a.getx()))
+    finally:
+        rapt.params.clear()
+        rapt.params.update(old)
+
+
+def test_mixed_check_flags_in_one_batch():
+    """rapt_amd.batch.advance with one tracer that checks adiabaticity and one
+    that does not (same field): the second reaches t0 + delta."""
+    f = F.CurrentSheet()
+    wl = W.current_sheet_protons(n=64, seed=2)
+    # a particle that becomes adiabatic early: find it among the sample
+    res = batch.advance_particles(f, np.c_[np.zeros(64), wl["pos"],
+                                           K.m_pr * wl["vel"] / np.sqrt(
+                                               1 - np.sum(wl["vel"] ** 2, 1) / C ** 2)[:, None]],
+                                  K.m_pr, K.e, 20.0, check=True)
+    i = int(np.nonzero(res.status == N.ST_SWITCH)[0][0])
+    a = rapt.Particle(wl["pos"][i], wl["vel"][i], 0.0, K.m_pr, K.e, f)
+    b = rapt.Particle(wl["pos"][i], wl["vel"][i], 0.0, K.m_pr, K.e, f)
+    a.check_adiabaticity = True
+    with pytest.raises(rapt.Adiabatic):
+        batch.advance([a, b], 20.0)
+    assert a.trajectory[-1, 0] < 20.0
+    assert b.tcur >= 20.0 and b.trajectory[-1, 0] >= 20.0
+    np.testing.assert_array_equal(b.trajectory[:len(a.trajectory)], a.trajectory)
+
+
+def _dipole_proton_row():
+    wl = W.dipole_protons_1mev(n=2)
+    g = 1 / np.sqrt(1 - wl["vel"][0] @ wl["vel"][0] / C ** 2)
+    return np.r_[0.0, wl["pos"][0], K.m_pr * g * wl["vel"][0]]
+
+
+def test_failure_statuses_match_oracle(oracle):
+    """Statuses forced from the host, each kernel exiting normally: nsteps
+    (a one-step limit with tight tolerances), step size too small (a start
+    at t = 1e12 s, where 0.1 |h| falls below |t| times the unit roundoff),
+    and a non-finite state (z = inf in a uniform field with the equatorial
+    constraint, so the step itself stays finite).  Rows, statuses and
+    counts as the oracle's."""
+    m, q = K.m_pr, K.e
+    last = _dipole_proton_row()
+    fe = F.EarthDipole()
+    fo = oracle.field_from_desc(fe.descriptor())
+    r = batch.advance_particles(fe, last, m, q, 1.0, nsteps=1,
+                                params=P(cyclotronresolution=20, solvertolerances=(1e-13, 1e-13)))
+    ro, io = oracle.particle_advance(fo, oracle.particle_params(
+        rtol=1e-13, atol=1e-13, nsteps=1, cyclotronresolution=20), last, m, q, 1.0)
+    assert r.status[0] == N.ST_NSTEPS == io["status"]
+    assert r.rows[0].shape == ro.shape and pos_err(r.rows[0], ro) < 1e-7
+    late = last.copy()
+    late[0] = 1e12
+    r = batch.advance_particles(fe, late, m, q, 1.0, params=P(cyclotronresolution=20))
+    ro, io = oracle.particle_advance(fo, oracle.particle_params(cyclotronresolution=20), late,
+                                     m, q, 1.0)
+    assert r.status[0] == N.ST_HSMALL == io["status"]
+    assert r.rows[0].shape == ro.shape
+    fu = F.UniformBz(Bz=1e-7)
+    row = np.r_[0.0, 1e5, 0.0, np.inf, m * 1e5, m * 2e5, 0.0]
+    Tc = 2 * np.pi * m / (q * 1e-7)
+    kw = dict(first_step=Tc / 200)
+    r = batch.advance_particles(fu, row, m, q, 1.0, row_cap=np.array([8]),
+                                params=P(cyclotronresolution=20, **{"enforce equatorial": True}),
+                                **kw)
+    ro, io = oracle.particle_advance(oracle.field_from_desc(fu.descriptor()),
+                                     oracle.particle_params(cyclotronresolution=20,
+                                                            enforce_equatorial=True, **kw),
+                                     row, m, q, 1.0)
+    assert r.status[0] == N.ST_NAN == io["status"]
+    assert r.rows[0].shape == ro.shape == (1, 7)
+    np.testing.assert_allclose(r.rows[0][:, [0, 1, 2, 4, 5]], ro[:, [0, 1, 2, 4, 5]],
+                               rtol=1e-12)
+    assert np.isinf(r.rows[0][0, 3])
+
+
+@pytest.mark.parametrize("tol,hdiv,hmin,bar", [(1e-3, 2.0, 1e-9, 1e-7),
+                                               (np.inf, 3.0, 0.0, 1e-10)])
+def test_rkf45_tracer_vs_oracle(oracle, tol, hdiv, hmin, bar):
+    """The RKF45 tracer stepper (rapt/rkf.py's control, once per output row) on
+    the device against the oracle's, whose per-row rkf calls are pinned
+    against the reference's rkf: 16 dipole protons for 2 s, adaptive at
+    tol 1e-3 (positions 1e-7) and in the fixed-step mode tol = inf (every
+    step accepted at hmax: positions and momenta 1e-10, equal counts)."""
+    m, q = K.m_pr, K.e
+    wl = W.dipole_protons_1mev(n=16)
+    g = 1 / np.sqrt(1 - np.sum(wl["vel"] ** 2, axis=1) / C ** 2)
+    last = np.c_[np.zeros(16), wl["pos"], (m * g)[:, None] * wl["vel"]]
+    f = F.EarthDipole()
+    fo = oracle.field_from_desc(f.descriptor())
+    Tc = np.array([oracle.cyclotron_period(fo, 0.0, wl["pos"][i], wl["vel"][i], m, q)
+                   for i in range(16)])
+    for i in range(0, 16, 4):
+        hmax = Tc[i] / 20 / hdiv
+        kw = dict(rkf_tol=tol, rkf_hmax=hmax, rkf_hmin=hmin)
+        r = batch.advance_particles(f, last[i], m, q, 2.0, stepper=N.STEP_RKF45,
+                                    params=P(cyclotronresolution=20), **kw)
+        ro, io = oracle.particle_advance(fo, oracle.particle_params(
+            stepper=oracle.RKF45, cyclotronresolution=20, **kw), last[i], m, q, 2.0)
+        assert r.status[0] == io["status"] == 1
+        assert r.rows[0].shape == ro.shape
+        assert pos_err(r.rows[0], ro) < bar
+        if tol == np.inf:
+            # dt = T_c / 20 of the device and the oracle can differ by an ulp,
+            # so an interval may end with an extra rounding-remainder step on
+            # one side: at most one per output row
+            assert abs(int(r.naccpt[0]) - io["naccpt"]) <= len(ro)
+            d = np.linalg.norm(r.rows[0][:, 4:7] - ro[:, 4:7], axis=1)
+            assert np.max(d / np.linalg.norm(ro[:, 4:7], axis=1)) < bar
+
+
+def test_save_load_mid_run_continues_identically(tmp_path):
+    """Particle and GuidingCenter saved after one advance and loaded into new
+    objects continue exactly as the originals."""
+    f = F.EarthDipole()
+    last = _dipole_proton_row()
+    mom = last[4:7]
+    v = mom / np.sqrt(K.m_pr ** 2 + mom @ mom / C ** 2)
+    old = dict(rapt.params)
+    rapt.params["GCtimestep"] = 0.05
+    try:
+        for t in (rapt.Particle(last[1:4], v, 0.0, K.m_pr, K.e, f),
+                  rapt.GuidingCenter(list(last[1:4]), 2e7, 35.0, None, 0.0, K.m_el, -K.e, f)):
+            t.advance(0.5)
+            fn = str(tmp_path / "t.pkl")
+            t.save(fn)
+            u = type(t)()
+            u.load(fn)
+            t.advance(0.5)
+            u.advance(0.5)
+            np.testing.assert_array_equal(u.trajectory, t.trajectory)
+            assert u.tcur == t.tcur
+    finally:
+        rapt.params.clear()
+        rapt.params.update(old)
+
+
+def test_device_accessors_match_reference(golden):
+    """Accessors that evaluate the field on the device, on the reference's
+    golden trajectories, against the reference's outputs: Particle.mu() and
+    guidingcenter() (fixed-point iteration per row; its tolerance 1e-3 makes
+    the reference's own answer depend on rounding only at the 1e-13 level),
+    GuidingCenter getB, getgamma, getBm, getke."""
+    A, Pz, G = golden("accessors"), golden("particles"), golden("gcs")
+    off = np.r_[0, np.cumsum(Pz["c1/len"])]
+    for k in range(3):
+        p = rapt.Particle()
+        p.mass, p.charge, p.field = K.m_pr, K.e, F.EarthDipole()
+        p.trajectory = Pz["c1/traj"][off[k]:off[k + 1]].copy()
+        np.testing.assert_allclose(p.mu(), A["p%d/mu" % k], rtol=1e-11)
+        gc = p.guidingcenter()
+        ref = A["p%d/guidingcenter" % k]
+        np.testing.assert_allclose(gc[:, :3], ref[:, :3], rtol=1e-11)
+        np.testing.assert_allclose(gc[:, 3:], ref[:, 3:], rtol=1e-11, atol=1e-9)
+    off = np.r_[0, np.cumsum(G["c2/len"])]
+    for k in range(3):
+        g = rapt.GuidingCenter()
+        g.mass, g.charge, g.field = K.m_el, -K.e, F.DoubleDipole()
+        g.mu = float(G["c2/mu"][k])
+        g.trajectory = G["c2/traj"][off[k]:off[k + 1]].copy()
+        for nm in ("getB", "getgamma", "getBm", "getke"):
+            np.testing.assert_allclose(getattr(g, nm)(), A["g%d/%s" % (k, nm)], rtol=1e-12,
+                                       err_msg=nm)
+
+
+# ---- mixed-precision configuration: EarthDipole + (0, 0, -10 nT) ----------------------
+
+def _config4(n):
+    wl = W.mixed_dipole_uniform(n=n)
+    f = F.DipolePlusUniform(Bu=(0.0, 0.0, wl["dBz"]))
+    m, q, v, pa = wl["mass"], wl["charge"], wl["v"], np.radians(wl["pa"])
+    g = 1 / np.sqrt(1 - (v / C) ** 2)
+    B0 = N.context().field_eval(f.descriptor(), "magB", np.c_[np.zeros(n), wl["pos"]])[:, 0]
+    ppar = g * m * v * np.cos(pa)
+    mu = g ** 2 * m * (v * np.sin(pa)) ** 2 / (2 * B0)
+    return wl, f, np.c_[np.zeros(n), wl["pos"], ppar], m, q, mu, v, g
+
+
+def _gc_energy(f, rows, m, mu):
+    B = N.context().field_eval(f.descriptor(), "magB", rows[:, :4])[:, 0]
+    mc = m * C
+    gam = np.sqrt(1 + 2 * mu * B / (mc * C) + (rows[:, 4] / mc) ** 2)
+    return (gam - 1) * mc * C
+
+
+def test_config4_gc_fp64_vs_oracle_and_fp32_per_species(oracle):
+    """256 protons and 256 electrons of the mixed-precision configuration as
+    guiding centres, advance(30), a row every second.
+
+    FP64 against the oracle (the reference's arithmetic): every proton within
+    1e-7 at every row, and the median electron too.  An adaptive trace that
+    is not bit-identical to the reference's leaves it where a step is accepted
+    on one side and rejected on the other; from there the two differ by the
+    integrator's own error.  For small-pitch electrons over 30 s that error is
+    ~1e-3 (the reference against a rtol 1e-12 run), so each particle beyond
+    1e-7 must be as close to that tight-tolerance trajectory as the oracle is
+    (within 5% of the oracle's distance).  Energy drift no larger than the
+    oracle's: the drifts (~1e-7) differ particle by particle by ~10% either
+    way, so the bar is the median per-particle ratio (at most 1.05) and no
+    particle above twice the oracle's drift (+1e-9).
+
+    FP32-field against FP64 with the same (I-)controller, per species:
+    position error median and maximum, and energy drift, within the bounds
+    measured on the full configuration."""
+    n = 512
+    wl, f, last5, m, q, mu, v, g = _config4(n)
+    fo = oracle.field_from_desc(f.descriptor())
+    prm = P()
+    a = batch.advance_gcs(f, last5, m, q, mu, v, 1.0, 30.0, params=prm, beta=-1.0)
+    b = batch.advance_gcs(f, last5, m, q, mu, v, 1.0, 30.0, params=prm, precision="fp32_field")
+    d64 = batch.advance_gcs(f, last5, m, q, mu, v, 1.0, 30.0, params=prm)
+    assert np.all(d64.status == N.ST_DONE) and np.all(a.status == N.ST_DONE)
+    assert np.all(b.status == N.ST_DONE)
+    cap = np.full(n, 40, np.int64)
+    rows_o, off, nrows, st, _ = oracle.batch_gc(fo, oracle.gc_params(), 0, last5, m, q, mu, v,
+                                                1.0, 30.0, cap, nthreads=8)
+    assert np.all(st == 1)
+    orow = [rows_o[off[i]:off[i] + nrows[i]] for i in range(n)]
+    e64 = np.array([pos_err(d64.rows[i], orow[i]) if d64.rows[i].shape == orow[i].shape
+                    else np.inf for i in range(n)])
+    far = np.nonzero(e64 > 1e-7)[0]
+    rows_t, off_t, nrows_t, st_t, _ = oracle.batch_gc(
+        fo, oracle.gc_params(rtol=1e-12, atol=1e-30, nsteps=10 ** 7), 0, last5[far], m[far],
+        q[far], mu[far], v[far], 1.0, 30.0, cap[far], nthreads=8)
+    assert np.all(st_t == 1)
+    for j, i in enumerate(far):
+        truth = rows_t[off_t[j]:off_t[j] + nrows_t[j]]
+        dg, do = pos_err(d64.rows[i], truth), pos_err(orow[i], truth)
+        assert dg <= 1.05 * do, (i, e64[i], dg, do)
+    stats = {}
+    for sp, name in ((0, "proton"), (1, "electron")):
+        idx = np.nonzero(wl["species"] == sp)[0]
+        drift64, drift_o, e32, drift32 = [], [], [], []
+        for i in idx:
+            K0 = (g[i] - 1) * m[i] * C * C
+            drift64.append(np.max(np.abs(_gc_energy(f, d64.rows[i], m[i], mu[i]) - K0)) / K0)
+            drift_o.append(np.max(np.abs(_gc_energy(f, orow[i], m[i], mu[i]) - K0)) / K0)
+            y, z = a.rows[i], b.rows[i]
+            assert y.shape == z.shape
+            e32.append(pos_err(z, y))
+            drift32.append(np.max(np.abs(_gc_energy(f, z, m[i], mu[i]) - K0)) / K0)
+        drift64, drift_o, e32 = np.array(drift64), np.array(drift_o), np.array(e32)
+        ratio = (drift64 + 1e-12) / (drift_o + 1e-12)
+        stats[name] = (e64[idx].max(), np.median(e64[idx]), np.mean(e64[idx] <= 1e-7),
+                       np.median(ratio), np.median(e32), e32.max(), np.median(drift32))
+        print("%s: fp64 vs oracle max %.2e median %.2e, within 1e-7 %.3f; energy drift ratio "
+              "median %.3f; fp32 vs fp64 median %.2e max %.2e; fp32 energy drift median %.2e"
+              % ((name,) + stats[name]))
+        assert np.all(drift64 <= 2 * drift_o + 1e-9), name
+    for name, (emax, emed, _, rmed, m32, x32, d32) in stats.items():
+        assert rmed <= 1.05, name
+        assert emed <= 1e-7, name
+        if name == "proton":
+            assert emax < 1e-7
+            assert m32 < 2e-4 and x32 < 1e-2 and d32 < 1e-4
+        else:
+            assert m32 < 1e-2 and x32 < 0.1 and d32 < 1e-3
+
+
+def test_config4_particle_subbatch_mu_vs_oracle(oracle):
+    """Particle sub-batch of the mixed-precision configuration (16 protons for
+    1 s, 16 electrons for 2 ms): FP64 positions against the oracle within
+    1e-7; the drift of the first invariant (Particle.mu(): guiding centre by
+    the fixed-point iteration, then gamma^2 m v_perp^2 / 2B) no larger than
+    the oracle's trajectory gives (+1e-5 of it: the two trajectories differ
+    by ~1e-8, which moves the gyrophase-dependent mu of a row by as much);
+    FP32-field against FP64 per species."""
+    n = 64
+    wl, f, _, m, q, _, _, g = _config4(n)
+    fo = oracle.field_from_desc(f.descriptor())
+    ctx = N.context()
+    for sp, delta, bar32 in ((0, 1.0, 1e-4), (1, 2e-3, 1e-5)):
+        # below 1 MeV, where the guiding-centre iteration converges on every row
+        sel = np.nonzero((wl["species"] == sp) & (wl["logK"] < 6.0))[0][:16]
+        last7 = np.c_[np.zeros(16), wl["pos"][sel], (m[sel] * g[sel])[:, None] * wl["vel"][sel]]
+        r64 = batch.advance_particles(f, last7, m[sel], q[sel], delta, params=P())
+        r32 = batch.advance_particles(f, last7, m[sel], q[sel], delta, params=P(),
+                                      precision="fp32_field", beta=0.0)
+        rb0 = batch.advance_particles(f, last7, m[sel], q[sel], delta, params=P(), beta=0.0)
+        for j, i in enumerate(sel):
+            ro, io = oracle.particle_advance(fo, oracle.particle_params(), last7[j], m[i], q[i],
+                                             delta)
+            x = r64.rows[j]
+            assert x.shape == ro.shape and pos_err(x, ro) < 1e-7
+            allg = np.vstack([last7[j:j + 1], x])
+            allo = np.vstack([last7[j:j + 1], ro])
+            mug = ctx.particle_gc(f.descriptor(), allg, m[i], q[i])[0][:, 5]
+            muo = []
+            for row in allo:
+                p = row[4:7]
+                vv = p / np.sqrt(m[i] ** 2 + p @ p / C ** 2)
+                R, vp, spd = oracle.guidingcenter(fo, row[0], row[1:4], vv, m[i], q[i])
+                muo.append(oracle.magnetic_moment(fo, row[0], R, vp, spd, m[i]))
+            muo = np.array(muo)
+            dg = np.max(np.abs(mug - mug[0])) / mug[0]
+            do = np.max(np.abs(muo - muo[0])) / muo[0]
+            assert dg <= do * (1 + 1e-5) + 1e-12, (sp, j, dg, do)
+            y, z = rb0.rows[j], r32.rows[j]
+            kk = min(len(y), len(z))
+            assert pos_err(z[:kk], y[:kk]) < bar32

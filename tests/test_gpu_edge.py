@@ -55,7 +55,7 @@ class _UniformGrid(F.Grid):
         x = np.linspace(-2.0, 2.0, 5) * K.Re
         sh = (5, 5, 5)
         return dict(time=0.0, x=x, y=x.copy(), z=x.copy(), Bx=np.zeros(sh), By=np.zeros(sh),
-                    Bz=np.full(sh, 1e-7))
+                    Bz=np.full(sh, 1e-7), Ex=np.zeros(sh), Ey=np.zeros(sh), Ez=np.zeros(sh))
 
 
 def test_field_line_integrals_refuse_untraceable_starts():

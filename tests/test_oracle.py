@@ -315,3 +315,50 @@ def test_planted_mistake_is_caught(oracle):
                            cyclotronresolution=20)
     n = min(len(got), len(ref))
     assert np.max(np.abs(got[:n, 1:4] - ref[:n, 1:4])) > 1e3
+
+
+@pytest.mark.parametrize("tol,hdiv,hmin", [(1e-3, 2.0, 1e-9), (np.inf, 3.0, 0.0)])
+def test_rkf_tracer_stepper_is_rkf_per_output_step(oracle, tol, hdiv, hmin):
+    """The RKF45 tracer stepper (a tracer advance with stepper RKF45) is
+    rapt/rkf.py's rkf(f, t, t + dt, y, tol, hmax, hmin) once per output row,
+    keeping the end point: bit-identical to orc_rkf (pinned against the
+    reference's rkf above) chained over the rows, adaptive and in the
+    fixed-step mode tol = inf (every step accepted, h = hmax; hmin = 0, since
+    the step that ends an interval can be a rounding remainder far below any
+    positive hmin, which rkf turns into its RuntimeError)."""
+    L = oracle.lib()
+    dp = C.POINTER(C.c_double)
+    L.orc_rhs_lorentz.argtypes = [C.POINTER(oracle.Field), C.POINTER(oracle.Params), C.c_double,
+                                  C.c_double, dp, C.c_double, dp, dp]
+    L.orc_rhs_lorentz.restype = None
+    f = F.EarthDipole()
+    fo = oracle.field_from_desc(f.descriptor())
+    from rapt_amd import workloads as W
+    wl = W.dipole_protons_1mev(n=2)
+    m, q = K.m_pr, K.e
+    g = 1 / np.sqrt(1 - wl["vel"][1] @ wl["vel"][1] / float(K.c) ** 2)
+    last = np.r_[0.0, wl["pos"][1], m * g * wl["vel"][1]]
+    Tc = oracle.cyclotron_period(fo, 0.0, last[1:4], wl["vel"][1], m, q)
+    hmax = Tc / 20 / hdiv
+    prm = oracle.particle_params(stepper=oracle.RKF45, cyclotronresolution=20, rkf_tol=tol,
+                                 rkf_hmax=hmax, rkf_hmin=hmin)
+    rows, info = oracle.particle_advance(fo, prm, last, m, q, 2.0)
+    assert info["status"] == 1 and len(rows) > 50
+    gm = C.c_double(np.sqrt(m ** 2 + last[4:] @ last[4:] / float(K.c) ** 2))
+
+    @RKF_CB
+    def cb(ctx, t, y, out):
+        L.orc_rhs_lorentz(C.byref(fo), C.byref(prm), m, q, C.byref(gm), t, y, out)
+    y, t = last[1:].copy(), last[0]
+    cap = 4096
+    tout, xout, npts = np.zeros(cap), np.zeros((cap, 6)), C.c_int64()
+    nacc = 0
+    for row in rows:
+        rc = L.orc_rkf(C.cast(cb, C.c_void_p), None, 6, t, row[0], oracle._dp(y), tol, hmax,
+                       hmin, oracle._dp(tout), oracle._dp(xout), cap, C.byref(npts))
+        assert rc == 1
+        k = npts.value
+        t, y = tout[k - 1], xout[k - 1].copy()
+        nacc += k - 1
+        np.testing.assert_array_equal(y, row[1:])
+    assert nacc == info["naccpt"]
