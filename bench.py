@@ -9,6 +9,14 @@ full advance(60) of the whole batch; the state is restored on the device
 between steps, so inputs stay resident in HBM.  A particle-step is one
 accepted integrator step of one particle.
 
+The same line carries, for the driver's one run: full-length parity of the
+timed rows against the oracle (the first --cpu-sample particles, the sample
+the CPU baseline integrates anyway), the rate with the library's default
+work order, the time to deliver the rows to the host, and summaries of the
+other configurations (Particle dipole batch with its own parity and first-
+invariant check, Adaptive current sheet, mixed precision, surrogate) under
+"configs" (--headline-only skips them).
+
 Prints one JSON line.  With --gpus > 1 it exits: this build measures one GPU.
 
 Other workloads (--workload, one JSON line each; not the default line):
@@ -36,6 +44,10 @@ sys.path.insert(0, ROOT)
 FLOP_RHS_TCB_DOUBLEDIPOLE = 484      # one Tao-Chan-Brizard RHS, 7 field evaluations
 FLOP_DOPRI5_STEP_N4 = 282            # stage combinations, error norm, control
 FLOP_RHS_TCB_REFERENCE = 795         # same RHS as the reference writes it (13 field evals)
+FLOP_RHS_LORENTZ_DIPOLE = 34         # one Lorentz RHS in EarthDipole (static gamma m)
+FLOP_DOP853_STEP_N6 = 988            # DOP853 attempt without its RHS: stage sums 666,
+                                     # stage times 23, solution + two error norms 282,
+                                     # norm combination 7, step-size control 10
 PEAK_FP64 = 78.6e12                  # MI355X FP64 vector (and matrix) peak, FLOP/s
 PEAK_HBM = 8.0e12                    # bytes/s
 ROW_BYTES_GC = 40                    # one (t, x, y, z, ppar) output row
@@ -77,7 +89,9 @@ def build_workload(n, device=0):
 
 def cpu_baseline(w, sample, threads):
     """The oracle (C restatement of the reference, OpenMP over particles) on
-    the first `sample` particles of the same workload."""
+    the first `sample` particles of the same workload.  Returns (accepted
+    steps, seconds, (rows, row offsets, row counts, status)) -- the rows are
+    the parity check's reference."""
     from oracle import oracle as O
     f = O.field_from_desc(w["field"].descriptor())
     prm = O.gc_params(rtol=1e-8, atol=1e-8)
@@ -88,7 +102,54 @@ def cpu_baseline(w, sample, threads):
                                                w["mu"][s], w["v"][s], w["dt"][s], w["delta"],
                                                cap, nthreads=threads)
     el = time.perf_counter() - t0
-    return cnt.naccpt, el
+    return cnt.naccpt, el, (rows, off, nrows, status)
+
+
+def _rel_pos_err(a, b):
+    if a.shape != b.shape:
+        return np.inf
+    if len(a) == 0:
+        return 0.0
+    d = np.linalg.norm(a[:, 1:4] - b[:, 1:4], axis=1)
+    return float(np.max(d / np.linalg.norm(b[:, 1:4], axis=1)))
+
+
+def parity_summary(gpu_rows, ref_rows, truth_fn=None, truth_max=64, bar=1e-7):
+    """Position parity of per-particle row arrays at the output times: the
+    largest relative error over each particle's rows, the share within `bar`,
+    and -- for up to truth_max particles beyond it, worst first -- how close
+    the device and the reference come to a tight-tolerance trajectory
+    (truth_fn(indices) -> list of row arrays).  An adaptive trace that is not
+    bit-identical to the reference's leaves it where a step is accepted on one
+    side and rejected on the other, and from there the two differ by the
+    integrator's own error; the ratio device/reference distance to the tight
+    trajectory then stays near 1."""
+    err = np.array([_rel_pos_err(g, r) for g, r in zip(gpu_rows, ref_rows)])
+    fin = err[np.isfinite(err)]
+    out = {"particles": int(len(err)), "rows": int(sum(len(r) for r in ref_rows)),
+           "row_count_mismatches": int(np.sum(~np.isfinite(err))),
+           "max_pos_rel_err": float(fin.max()) if fin.size else None,
+           "median_pos_rel_err": float(np.median(fin)) if fin.size else None,
+           "p99_pos_rel_err": float(np.percentile(fin, 99)) if fin.size else None,
+           "frac_within_%g" % bar: float(np.mean(err <= bar)), "bar": bar}
+    far = np.nonzero(err > bar)[0]
+    out["beyond_bar"] = int(far.size)
+    if far.size and truth_fn is not None:
+        sel = far[np.argsort(-err[far])][:truth_max]
+        truth = truth_fn(sel)
+        dg = np.array([_rel_pos_err(gpu_rows[i], t) for i, t in zip(sel, truth)])
+        do = np.array([_rel_pos_err(ref_rows[i], t) for i, t in zip(sel, truth)])
+        r = dg / do
+        out["beyond_bar_vs_tight_tolerance"] = {
+            "checked": int(sel.size), "tight_rtol": 1e-12,
+            "reference_dist_median": float(np.median(do)),
+            "device_dist_median": float(np.median(dg)),
+            "device_over_reference_median": float(np.median(r)),
+            "device_over_reference_max": float(np.max(r)),
+            "worst": [{"particle": int(i), "pos_rel_err_vs_reference": float(err[i]),
+                       "device_dist": float(a), "reference_dist": float(b)}
+                      for i, a, b in list(zip(sel, dg, do))[:4]]}
+    return out
 
 def timed_run(field, mode, S, flags, cap, desc, steps, warmup, width, rows=False, cost=None):
     """Upload once, then warmup + timed runs of the same batch (state reset on
@@ -142,10 +203,16 @@ def base_line(metric, value, unit, args, el, dtype, config):
             "data": "synthetic (seeded)", "config": config}
 
 
-def run_particle_dipole(args):
-    """1024 protons at 1 MeV in EarthDipole, Particle.advance(10), DOP853."""
+def run_particle_dipole(args, parity=None):
+    """1024 protons at 1 MeV in EarthDipole, Particle.advance(10), DOP853.
+    With parity: every proton's rows against the oracle, and the drift of the
+    first invariant (mu of each row: the guiding centre by the fixed-point
+    iteration, then gamma^2 m v_perp^2 / 2B, rapt_particle_gc on the device
+    for both trajectories) against the oracle's trajectory."""
     import rapt_amd as rapt
     from rapt_amd import _native as N, batch as BT, workloads as W
+    if parity is None:
+        parity = not args.no_cpu
     n = args.n or 1024
     wl = W.dipole_protons_1mev(n=n)
     f = rapt.fields.EarthDipole()
@@ -156,19 +223,76 @@ def run_particle_dipole(args):
     S = BT.particle_state(last7, rapt.m_pr, rapt.e)
     dt = BT.particle_dt(f, last7, rapt.m_pr, rapt.e, 20)
     cap = BT._rows_needed(10.0, dt)
-    desc = N.make_desc(N.MODE_PARTICLE, dict(rapt.params), 10.0, lanes=args.lanes,
-                       block=args.block)
+    desc = N.make_desc(N.MODE_PARTICLE, dict(rapt.params, cyclotronresolution=20), 10.0,
+                       lanes=args.lanes, block=args.block)
     tot, el, out = timed_run(f, N.MODE_PARTICLE, S, np.zeros(n, np.int32), cap, desc,
-                             args.steps, args.warmup, 7)
+                             args.steps, args.warmup, 7, rows=parity)
     value = tot["naccpt"] / el
     res = base_line("FP64 particle-steps/s (1 GPU)", value, "particle-steps/s", args, el, "f64",
                     {"workload": "EarthDipole Particle batch: %d protons, 1 MeV, r U[4,7] Re, "
                                  "pitch U[10,90] deg; DOP853 default tolerances, advance(10), "
                                  "dt=T_c/20" % n, "particles": n})
-    res["accepted_steps_per_step"] = tot["naccpt"] / args.steps
-    res["kernel_ms_avg"] = tot["kernel_ms"] / args.steps
+    launches = args.steps
+    ksec = tot["kernel_ms"] / launches / 1e3
+    flop = (tot["nfcn"] * FLOP_RHS_LORENTZ_DIPOLE + tot["nstep"] * FLOP_DOP853_STEP_N6) / launches
+    res["accepted_steps_per_step"] = tot["naccpt"] / launches
+    res["rhs_evals_per_step"] = tot["nfcn"] / launches
+    res["kernel_ms_avg"] = tot["kernel_ms"] / launches
     res["particles_done"] = int(np.sum(out["status"] == 1))
-    print(json.dumps(res), flush=True)
+    res["roofline"] = {"bound": "valu_fp64", "achieved": flop / ksec / 1e12,
+                       "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                       "frac": flop / ksec / PEAK_FP64, "traffic": None,
+                       "flop_per_launch": flop,
+                       "flop_per_particle_step": flop / (tot["naccpt"] / launches),
+                       "note": "%d lanes are %.3f waves per SIMD: the launch is the longest "
+                               "trace's dependent chain" % (n, n / 64 / 1024)}
+    if parity:
+        from oracle import oracle as O
+        fo = O.field_from_desc(f.descriptor())
+        threads = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        rows_o, off_o, nrows_o, st_o, cnt_o = O.batch_particle(
+            fo, O.particle_params(cyclotronresolution=20), last7, rapt.m_pr, rapt.e, 10.0, cap,
+            nthreads=threads)
+        t_cpu = time.perf_counter() - t0
+        ref = [rows_o[off_o[i]:off_o[i] + nrows_o[i]] for i in range(n)]
+        gpu = [out["rows"][out["row_off"][i]:out["row_off"][i] + out["nrows"][i]]
+               for i in range(n)]
+
+        def truth(sel):
+            r, o, k, st, _ = O.batch_particle(
+                fo, O.particle_params(cyclotronresolution=20, rtol=1e-12, atol=1e-30,
+                                      nsteps=10 ** 7), last7[sel], rapt.m_pr, rapt.e, 10.0,
+                cap[sel], nthreads=threads)
+            return [r[o[j]:o[j] + k[j]] for j in range(len(sel))]
+        res["parity"] = parity_summary(gpu, ref, truth)
+        # first invariant along both trajectories, start row included
+        ctx = N.context(0)
+        allg = np.vstack([np.vstack([last7[i:i + 1], gpu[i]]) for i in range(n)])
+        allo = np.vstack([np.vstack([last7[i:i + 1], ref[i]]) for i in range(n)])
+        cg = np.r_[0, np.cumsum([len(x) + 1 for x in gpu])]
+        co = np.r_[0, np.cumsum([len(x) + 1 for x in ref])]
+        mg, sg = ctx.particle_gc(f.descriptor(), allg, rapt.m_pr, rapt.e)
+        mo, so = ctx.particle_gc(f.descriptor(), allo, rapt.m_pr, rapt.e)
+        dg = np.array([np.max(np.abs(mg[cg[i]:cg[i + 1], 5] / mg[cg[i], 5] - 1))
+                       for i in range(n)])
+        do = np.array([np.max(np.abs(mo[co[i]:co[i + 1], 5] / mo[co[i], 5] - 1))
+                       for i in range(n)])
+        ok = np.isfinite(dg) & np.isfinite(do)
+        rr = dg[ok] / do[ok]
+        res["mu_drift"] = {"particles": int(ok.sum()),
+                           "device_median": float(np.median(dg[ok])),
+                           "reference_median": float(np.median(do[ok])),
+                           "device_over_reference_median": float(np.median(rr)),
+                           "device_over_reference_max": float(np.max(rr)),
+                           "frac_not_larger": float(np.mean(dg[ok] <= do[ok] * (1 + 1e-9))),
+                           "gc_iteration_failures": int(np.sum(sg < 0) + np.sum(so < 0))}
+        res["cpu_baseline"] = {"value": cnt_o.naccpt / t_cpu, "unit": "particle-steps/s",
+                               "cores": threads, "kind": "port",
+                               "sample": "all %d protons, full advance(10), C oracle with "
+                                         "OpenMP (%.1f s)" % (n, t_cpu)}
+        res["gpu_vs_cpu"] = value / (cnt_o.naccpt / t_cpu)
+    return res
 
 
 def run_grid(args):
@@ -250,7 +374,7 @@ def run_grid(args):
                      "algorithmic_bytes_per_launch": gather,
                      "note": "gather bytes if every corner read came from HBM; L2/MALL "
                              "reuse between nearby evaluations lowers the real traffic"}})
-    print(json.dumps(res), flush=True)
+    return res
 
 
 def run_adaptive_sheet(args):
@@ -287,7 +411,7 @@ def run_adaptive_sheet(args):
     res["kernel_ms_avg"] = tot["kernel_ms"] / args.steps
     st = out["status"]
     res["status_counts"] = {str(k): int(np.sum(st == k)) for k in np.unique(st)}
-    print(json.dumps(res), flush=True)
+    return res
 
 
 def _pct(a):
@@ -418,7 +542,7 @@ def run_mixed_precision(args):
                                  "advance(30), output every 1 s" % n, "particles": n})
     res["gc_runs"] = res_gc
     res["errors_per_species"] = errs
-    print(json.dumps(res), flush=True)
+    return res
 
 
 
@@ -516,7 +640,7 @@ def run_surrogate(args):
                        "frac": flop / (kms / 1e3) / 2.5e15, "traffic": None,
                        "flop_per_query": 4 * 265216}
     s.close()
-    print(json.dumps(res), flush=True)
+    return res
 
 
 def main():
@@ -546,6 +670,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (use --lpp/--lanes as given)")
     ap.add_argument("--cpu-sample", type=int, default=8192)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the other configurations' summaries in the default line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", choices=("cost", "dt", "input"), default="cost",
                     help="library work order: descending cost measured by a trial launch "
@@ -571,11 +697,51 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe}))
         return
 
-    if args.workload != "gc_doubledipole":
-        {"particle_dipole": run_particle_dipole, "adaptive_sheet": run_adaptive_sheet,
-         "grid": run_grid,
-         "mixed_precision": run_mixed_precision, "surrogate": run_surrogate}[args.workload](args)
-        return
+    runners = {"particle_dipole": run_particle_dipole, "adaptive_sheet": run_adaptive_sheet,
+               "grid": run_grid, "mixed_precision": run_mixed_precision,
+               "surrogate": run_surrogate, "gc_doubledipole": run_gc_doubledipole}
+    res = runners[args.workload](args)
+    if args.workload == "gc_doubledipole" and not args.headline_only and not args.top:
+        res["configs"] = other_configs(args)
+    print(json.dumps(res), flush=True)
+
+
+def other_configs(args):
+    """Short runs of the other configurations for the same JSON line (each
+    its own seeded workload at full size; a failure is recorded, not raised)."""
+    import copy
+    out = {}
+    for name, fn, keys in (
+            ("particle_dipole", run_particle_dipole,
+             ("value", "ms_per_step", "kernel_ms_avg", "accepted_steps_per_step", "roofline",
+              "parity", "mu_drift", "cpu_baseline", "gpu_vs_cpu", "particles_done", "config")),
+            ("adaptive_sheet", run_adaptive_sheet,
+             ("value", "ms_per_step", "kernel_ms_avg", "accepted_steps_per_step",
+              "switches_per_step", "status_counts", "config")),
+            ("mixed_precision", run_mixed_precision,
+             ("value", "gc_runs", "errors_per_species", "config")),
+            ("surrogate", run_surrogate,
+             ("value", "unit", "speedup_vs_traced", "traced_queries_per_s", "inference_kernel_ms",
+              "fallback_queries", "fallback_ms", "heldout_surrogate_rel_err",
+              "heldout_with_fallback_rel_err", "heldout_flagged_frac", "roofline", "train_s",
+              "config"))):
+        a = copy.copy(args)
+        a.n, a.lanes, a.block = 0, 0, 0
+        a.steps, a.warmup = min(args.steps, 3), 1
+        t0 = time.perf_counter()
+        try:
+            r = fn(a)
+            out[name] = {k: r[k] for k in keys if k in r}
+            out[name]["metric"] = r.get("metric")
+        except Exception as e:  # keep the headline line whatever an extra does
+            out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+        out[name]["wall_s"] = time.perf_counter() - t0
+        log("%s: %.1f s" % (name, out[name]["wall_s"]))
+    return out
+
+
+def run_gc_doubledipole(args):
+    """The headline: the DoubleDipole guiding-centre batch (module docstring)."""
     args.n = args.n or 65536
     from rapt_amd import _native as N
     w = build_workload(args.n)
@@ -601,11 +767,14 @@ def main():
         log("top %d particles: %.0f-%.0f accepted steps each" % (n, full[idx].min(),
                                                                  full[idx].max()))
     cost = None
+    cost_s = None
     if args.order == "cost":
         from rapt_amd import autotune as AT
+        t0 = time.perf_counter()
         cost = AT.trial_cost(w["field"], S, flags, cap, desc, trial_fraction=0.5)
+        cost_s = time.perf_counter() - t0
         log("work order: descending accepted steps of a trial launch over half the "
-            "advance (mean %.0f per particle)" % cost.mean())
+            "advance (mean %.0f per particle, %.2f s)" % (cost.mean(), cost_s))
     tuned = None
     if not args.no_tune:
         from rapt_amd import autotune as AT
@@ -619,36 +788,56 @@ def main():
                  "cost_trial_fraction_of_delta": 0.5 if args.order == "cost" else None,
                  "trial_repeats": 2}
         log("autotune chose %s" % shape)
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+
+    def timed(b, steps, warmup, label):
+        for i in range(warmup):
+            b.reset()
+            c = b.run(desc)
+            log("%s warmup %d: %.1f ms, %d accepted steps" % (label, i, c.kernel_ms, c.naccpt))
+        hip.hipDeviceSynchronize()
+        tot = dict(naccpt=0, nrejct=0, nstep=0, nfcn=0, kernel_ms=0.0)
+        t0 = time.perf_counter()
+        for i in range(steps):
+            b.reset()
+            c = b.run(desc)
+            for k in ("naccpt", "nrejct", "nstep", "nfcn"):
+                tot[k] += getattr(c, k)
+            tot["kernel_ms"] += c.kernel_ms
+            log("%s step %d: %.1f ms, %d accepted steps" % (label, i, c.kernel_ms, c.naccpt))
+        hip.hipDeviceSynchronize()
+        return tot, time.perf_counter() - t0
+
     b = N.Batch(ctx, n)
     b.upload(S, flags)
     if cost is not None:
         b.set_cost(cost)
     b.set_rows(5, cap)
     log("rows allocated: %d (%.2f GB)" % (cap.sum(), cap.sum() * 40 / 1e9))
-
-    for i in range(args.warmup):
-        b.reset()
-        c = b.run(desc)
-        log("warmup %d: %.1f ms, %d accepted steps" % (i, c.kernel_ms, c.naccpt))
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipDeviceSynchronize()
-    tot = dict(naccpt=0, nrejct=0, nstep=0, nfcn=0, kernel_ms=0.0)
+    tot, el = timed(b, args.steps, args.warmup, "timed")
+    # rows of the last timed advance delivered to the host (the reference's
+    # advance hands back host trajectories): timed apart from the device rate
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        b.reset()
-        c = b.run(desc)
-        for k in ("naccpt", "nrejct", "nstep", "nfcn"):
-            tot[k] += getattr(c, k)
-        tot["kernel_ms"] += c.kernel_ms
-        log("step %d: %.1f ms, %d accepted steps" % (i, c.kernel_ms, c.naccpt))
-    hip.hipDeviceSynchronize()
-    el = time.perf_counter() - t0
-    out = b.download(rows=False)
+    out = b.download(rows=not args.no_cpu)
+    dl_s = time.perf_counter() - t0
+    out["row_off"] = np.asarray(b.row_off, np.int64).copy()
     ok = int(np.sum(out["status"] == 1))
     q = np.percentile(out["naccpt"], [50, 90, 99, 100])
     log("accepted steps per particle: mean %.0f, p50 %.0f, p90 %.0f, p99 %.0f, max %.0f"
         % (out["naccpt"].mean(), *q))
+    # the library's default work order (shortest output step first), same shape
+    dflt = None
+    if args.order == "cost" and not args.top:
+        b2 = N.Batch(ctx, n)
+        try:
+            b2.upload(S, flags)
+            b2.set_rows(5, cap)
+            t2, e2 = timed(b2, min(args.steps, 5), 1, "output-step order")
+            dflt = {"value": t2["naccpt"] / e2, "ms_per_step": e2 / min(args.steps, 5) * 1e3,
+                    "kernel_ms_avg": t2["kernel_ms"] / min(args.steps, 5)}
+        finally:
+            b2.close()
     value = tot["naccpt"] / el
     launches = args.steps
     flop_launch = (tot["nfcn"] * FLOP_RHS_TCB_DOUBLEDIPOLE +
@@ -685,8 +874,14 @@ def main():
         "rhs_evals_per_step": tot["nfcn"] / launches,
         "rejected_steps_per_step": tot["nrejct"] / launches,
         "kernel_ms_avg": tot["kernel_ms"] / launches,
+        "cost_trial_s": cost_s,
+        "value_output_step_order": dflt,
+        "row_delivery": {"rows": rows_launch, "bytes": rows_launch * ROW_BYTES_GC,
+                         "download_s": dl_s if not args.no_cpu else None,
+                         "value_incl_download": (tot["naccpt"] / launches) /
+                         (el / launches + dl_s) if not args.no_cpu else None},
         "roofline": {
-            "bound": "mfma", "compute": "FP64 vector ALU (78.6 TFLOP/s, same as FP64 matrix)",
+            "bound": "valu_fp64", "compute": "FP64 vector ALU (78.6 TFLOP/s)",
             "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64, "traffic": PROFILED_TRAFFIC.get(n),
             "flop_per_launch": flop_launch,
@@ -700,15 +895,34 @@ def main():
     b.close()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        steps_cpu, el_cpu = cpu_baseline(w, min(args.cpu_sample, n), threads)
+        sample = min(args.cpu_sample, n)
+        steps_cpu, el_cpu, (rows_o, off_o, nrows_o, st_o) = cpu_baseline(w, sample, threads)
         res["cpu_baseline"] = {
             "value": steps_cpu / el_cpu, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
             "sample": "first %d particles of the same workload, full advance(60), C oracle "
-                      "with OpenMP (%.1f s)" % (min(args.cpu_sample, n), el_cpu),
+                      "with OpenMP (%.1f s)" % (sample, el_cpu),
         }
         res["gpu_vs_cpu"] = value / (steps_cpu / el_cpu)
-    print(json.dumps(res), flush=True)
+        from oracle import oracle as O
+        fo = O.field_from_desc(w["field"].descriptor())
+        ref = [rows_o[off_o[i]:off_o[i] + nrows_o[i]] for i in range(sample)]
+        gpu = [out["rows"][out["row_off"][i]:out["row_off"][i] + out["nrows"][i]]
+               for i in range(sample)]
+
+        def truth(sel):
+            c = (np.ceil(w["delta"] / w["dt"][sel]) + 3).astype(np.int64)
+            r, o, k, st, _ = O.batch_gc(fo, O.gc_params(rtol=1e-12, atol=1e-30, nsteps=10 ** 7),
+                                        0, w["last5"][sel], w["m"], w["q"], w["mu"][sel],
+                                        w["v"][sel], w["dt"][sel], w["delta"], c,
+                                        nthreads=threads)
+            return [r[o[j]:o[j] + k[j]] for j in range(len(sel))]
+        t0 = time.perf_counter()
+        res["parity"] = parity_summary(gpu, ref, truth)
+        res["parity"]["what"] = ("rows of the last timed advance against the oracle's, the "
+                                 "CPU baseline's %d particles, full advance(60)" % sample)
+        log("parity: %.1f s" % (time.perf_counter() - t0))
+    return res
 
 
 if __name__ == "__main__":
