@@ -363,7 +363,7 @@ def test_config4_gc_fp64_vs_oracle_and_fp32_per_species(oracle):
     is not bit-identical to the reference's leaves it where a step is accepted
     on one side and rejected on the other; from there the two differ by the
     integrator's own error.  For small-pitch electrons over 30 s that error is
-    ~1e-3 (the reference against a rtol 1e-12 run), so each particle beyond
+    ~1e-3 (the reference against a rtol 1e-11 run), so each particle beyond
     1e-7 must be as close to that tight-tolerance trajectory as the oracle is
     (within 5% of the oracle's distance).  Energy drift no larger than the
     oracle's: the drifts (~1e-7) differ particle by particle by ~10% either
