@@ -325,6 +325,26 @@ int rapt_batch_epoch_next(rapt_batch *b, const double **rows, const int64_t **nr
                           const int32_t **status, int32_t *running, rapt_counters *counters);
 int rapt_batch_epochs_end(rapt_batch *b);
 
+/* Allocation-free advance on caller-owned device buffers.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels as rapt_batch_run; every pointer except d and
+ * cnt is device memory of the context's device: state [RAPT_NSLOT][n]
+ * (slot-major, updated in place), flags [n] (in/out), row_off/row_cap [n]
+ * (rows of particle i at rows + width * row_off[i], at most row_cap[i] of
+ * them; width 7/5/8 by mode), rows, nrows/status/naccpt [n] (out), order [n]
+ * (optional work order, particle indices; NULL: input order).  The launch
+ * calls no hipMalloc/hipFree: its work-queue and step counters live in a
+ * 64-byte scratch array that each device holds until the process ends, so
+ * calls that overlap on one device race (serialize them); each call clears
+ * the counters first.  Runs on the context's stream and returns when the
+ * advance is done; cnt (host, may be NULL) as rapt_batch_run. */
+int rapt_trace_device_noalloc(rapt_ctx *ctx, const rapt_trace_desc *desc, int64_t n,
+                              double *state, int32_t *flags, const int64_t *row_off,
+                              const int64_t *row_cap, double *rows, int64_t *nrows,
+                              int32_t *status, int64_t *naccpt, const int64_t *order,
+                              rapt_counters *counters);
+
 /* One-shot convenience: upload, run, download (host buffers). */
 int rapt_trace(rapt_ctx *ctx, const rapt_trace_desc *desc, int64_t n, double *state,
                int32_t *flags, const int64_t *row_cap, double *rows, int64_t *nrows,

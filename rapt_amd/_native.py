@@ -121,6 +121,9 @@ def lib():
     L.rapt_batch_download.argtypes = [vp, dp, i32p, i32p, i64p, dp, i64p]
     L.rapt_trace.argtypes = [vp, C.POINTER(TraceDesc), C.c_int64, dp, i32p, i64p, dp, i64p,
                              i32p, C.POINTER(Counters)]
+    # device pointers (as integers) for the allocation-free entry point
+    L.rapt_trace_device_noalloc.argtypes = [vp, C.POINTER(TraceDesc), C.c_int64, vp, vp, vp,
+                                            vp, vp, vp, vp, vp, vp, C.POINTER(Counters)]
     for nm in ("rapt_ctx_create", "rapt_ctx_destroy", "rapt_field_set", "rapt_field_eval",
                "rapt_halfbounce", "rapt_bounce_labels", "rapt_bounce_center_advance",
                "rapt_batch_epochs_begin", "rapt_batch_epoch_next", "rapt_batch_epochs_end",
@@ -130,7 +133,7 @@ def lib():
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
                "rapt_batch_reset", "rapt_batch_set_cost", "rapt_eye", "rapt_particle_gc",
-               "rapt_grid_set"):
+               "rapt_grid_set", "rapt_trace_device_noalloc"):
         getattr(L, nm).restype = C.c_int
     _lib = L
     return L

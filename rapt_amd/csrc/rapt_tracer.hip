@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <math.h>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -145,6 +146,41 @@ void rapt_launch_gcn(hipStream_t st, int model, int lanes_per_particle, const KA
 void rapt_launch_trace_f32(hipStream_t st, int model, int kind, const KArgs& A, int block,
                            int64_t lanes);
 
+// ---- scratch of the allocation-free entry point -----------------------------
+// The work-queue counter(s) and step counters that rapt_batch_run keeps in the
+// batch's d_work: 8 words (queue 0, accepted, rejected, attempts, RHS
+// evaluations, switches, the mixed layouts' queues 1 and 2).  One copy per
+// device, held until the process ends; host code uses the address that
+// hipGetSymbolAddress gives for the current device, never this name.
+#define RAPT_NOALLOC_WORDS 8
+static __device__ __align__(256) unsigned long long scratch[RAPT_NOALLOC_WORDS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (current), looked up once per
+// device.  Returns nullptr when the lookup fails.
+unsigned long long* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    g_scratch[device] = static_cast<unsigned long long*>(p);
+  }
+  return g_scratch[device];
+}
+}  // namespace
+
+// clears n words (a fill kernel rather than a memset node, so a caller may
+// capture the entry point in a graph)
+__global__ void k_fill_u64(unsigned long long* p, int n) {
+  const int i = (int)threadIdx.x;
+  if (i < n) p[i] = 0ULL;
+}
+
 template <int M, int K>
 static void launch_trace(rapt_ctx* c, const KArgs& A, int block, int64_t lanes) {
   const int64_t g = (lanes + block - 1) / block;
@@ -187,6 +223,18 @@ int rapt_ctx_create(int device, rapt_ctx** out) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return RAPT_E_HIP;
+  }
+  // this device's copy of the allocation-free scratch (device current)
+  c->noalloc_scratch = scratch_address(device);
+  if (!c->noalloc_scratch) {
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return RAPT_E_HIP;
   }
@@ -756,6 +804,68 @@ int rapt_batch_download(rapt_batch* b, double* state, int32_t* flags, int32_t* s
     HIP_OK(c, hipMemcpyAsync(rows, b->d_rows, sizeof(double) * b->width * b->total_rows,
                              hipMemcpyDeviceToHost, c->stream));
   HIP_OK(c, hipStreamSynchronize(c->stream));
+  return RAPT_OK;
+}
+
+int rapt_trace_device_noalloc(rapt_ctx* c, const rapt_trace_desc* d, int64_t n,
+                              double* state, int32_t* flags, const int64_t* row_off,
+                              const int64_t* row_cap, double* rows, int64_t* nrows,
+                              int32_t* status, int64_t* naccpt, const int64_t* order,
+                              rapt_counters* cnt) {
+  if (!c || !d) return RAPT_E_ARG;
+  if (n <= 0 || !state || !flags || !row_off || !row_cap || !rows || !nrows || !status ||
+      !naccpt)
+    return set_err(c, RAPT_E_ARG, "noalloc: n > 0 and every device buffer are required");
+  if (!c->noalloc_scratch) return set_err(c, RAPT_E_STATE, "noalloc: no scratch on this device");
+  // the kernels use counters 0-7 (queue 0, five step counters, the mixed
+  // layouts' queues 1 and 2): the scratch must hold them all
+  static_assert(RAPT_NOALLOC_WORDS >= 8, "scratch too small for the work counters");
+  // a view of the caller's buffers with the batch layout the launch code reads
+  // (nothing here is owned or freed)
+  rapt_batch view;
+  view.ctx = c;
+  view.n = n;
+  view.d_state = state;
+  view.d_flags = flags;
+  view.d_status = status;
+  view.d_nrows = nrows;
+  view.d_naccpt = naccpt;
+  view.d_rowoff = const_cast<int64_t*>(row_off);
+  view.d_rowcap = const_cast<int64_t*>(row_cap);
+  view.d_order = const_cast<int64_t*>(order);
+  view.d_rows = rows;
+  view.width = d->mode == RAPT_MODE_PARTICLE ? 7 : (d->mode == RAPT_MODE_GC ? 5 : 8);
+  view.d_work = c->noalloc_scratch;
+  rapt_trace_desc dd = *d;
+  if (!order) dd.work_order = 1;  // no order given: input order
+  KArgs A;
+  int block = 256;
+  int64_t lanes = 0;
+  bool grouped = false;
+  const int rc = prepare_run(&view, &dd, A, block, lanes, grouped);
+  if (rc) return rc;
+  // the scratch keeps the last call's counters: clear them
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->noalloc_scratch,
+                     (int)RAPT_NOALLOC_WORDS);
+  HIP_OK(c, hipGetLastError());
+  HIP_OK(c, hipEventRecord(c->ev0, c->stream));
+  launch_tracer(&view, &dd, A, block, lanes, grouped);
+  HIP_OK(c, hipGetLastError());
+  HIP_OK(c, hipEventRecord(c->ev1, c->stream));
+  unsigned long long w[RAPT_NOALLOC_WORDS];
+  HIP_OK(c, hipMemcpyAsync(w, c->noalloc_scratch, sizeof w, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  if (cnt) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    cnt->naccpt = (int64_t)w[1];
+    cnt->nrejct = (int64_t)w[2];
+    cnt->nstep = (int64_t)w[3];
+    cnt->nfcn = (int64_t)w[4];
+    cnt->nswitch = (int64_t)w[5];
+    cnt->nrows = 0;
+    cnt->kernel_ms = ms;
+  }
   return RAPT_OK;
 }
 
