@@ -649,7 +649,7 @@ def main():
     ap.add_argument("--workload", default="gc_doubledipole",
                     choices=("gc_doubledipole", "particle_dipole", "adaptive_sheet",
                              "mixed_precision", "surrogate", "grid"))
-    ap.add_argument("--epochs", type=int, default=60, help="surrogate training epochs")
+    ap.add_argument("--epochs", type=int, default=100, help="surrogate training epochs")
     ap.add_argument("--dump", default="", help="surrogate: write held-out results to this .npz")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)

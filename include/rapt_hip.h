@@ -264,6 +264,17 @@ int rapt_mlp_create(rapt_ctx *ctx, int32_t members, const float *theta0, double 
 int rapt_mlp_destroy(rapt_mlp *m);
 /* output normalisation: shift[3] then scale[3] (scale > 0) of the residuals */
 int rapt_mlp_set_norm(rapt_mlp *m, const float *shift_scale);
+/* Regime of the drift label: the reference's gradI differences I across the
+ * field line (rapt/flutils.py:196-227) and falls back to a one-sided
+ * difference where the inner line (L Re - eyegradientstep on the query's x
+ * axis) has no mirror point -- a jump of O(eyegradientstep / L) in omega_d
+ * across the curve B(L Re - eyegradientstep) = B_m.  With eyegradientstep > 0
+ * the features of the two sides move apart (the pitch-angle feature by 2.5)
+ * so each side is a smooth function for the network, and rapt_mlp_infer flags
+ * queries within 1e-5 of the curve for the traced fallback; 0 turns it off.
+ * Uses the context's field at the time of the call (dipole-type models 1, 2,
+ * 5, 8); applies to rapt_mlp_targets and rapt_mlp_infer. */
+int rapt_mlp_set_split(rapt_mlp *m, double eyegradientstep, double Re);
 /* features X [n][3] and targets T [n][3] from queries [n][3] and traced labels
  * [n][3] (rapt_bounce_labels); normalised != 0 applies the shift/scale. */
 int rapt_mlp_targets(rapt_mlp *m, int64_t n, const float *query, const double *labels,

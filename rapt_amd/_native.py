@@ -100,6 +100,7 @@ def lib():
     L.rapt_mlp_create.argtypes = [vp, C.c_int32, fp, C.c_double, C.c_double, C.POINTER(vp)]
     L.rapt_mlp_destroy.argtypes = [vp]
     L.rapt_mlp_set_norm.argtypes = [vp, fp]
+    L.rapt_mlp_set_split.argtypes = [vp, C.c_double, C.c_double]
     L.rapt_mlp_targets.argtypes = [vp, C.c_int64, fp, dp, C.c_int, fp, fp]
     L.rapt_mlp_train.argtypes = [vp, C.c_int64, fp, fp, C.c_int32, C.c_int32, i64p,
                                  C.c_float, C.c_float, dp, fp]
@@ -128,7 +129,7 @@ def lib():
                "rapt_halfbounce", "rapt_bounce_labels", "rapt_bounce_center_advance",
                "rapt_batch_epochs_begin", "rapt_batch_epoch_next", "rapt_batch_epochs_end",
                "rapt_mlp_create", "rapt_mlp_destroy",
-               "rapt_mlp_set_norm", "rapt_mlp_targets", "rapt_mlp_train", "rapt_mlp_get_params",
+               "rapt_mlp_set_norm", "rapt_mlp_set_split", "rapt_mlp_targets", "rapt_mlp_train", "rapt_mlp_get_params",
                "rapt_mlp_forward", "rapt_mlp_infer",
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",

@@ -26,6 +26,7 @@
 #include <new>
 
 #include "../../include/rapt_hip.h"
+#include "rapt_fields.hpp"
 #include "rapt_internal.hpp"
 
 namespace rapt {
@@ -101,9 +102,52 @@ __device__ __forceinline__ Base baseline(float L, float adeg, float lgk, double 
   return Base{(float)log(Tb), (float)log(6.283185307179586 / Td), (float)log(lat)};
 }
 
-__device__ __forceinline__ void features(const float* q, float x[3]) {
+// ---- the two regimes of the drift label ------------------------------------------------
+// The reference's gradI (rapt/flutils.py:196-227) differences I on lines
+// eyegradientstep d either side of the query point; on the query's x axis the
+// inner line starts at (L Re - d, 0, 0).  When the equatorial |B| of that line
+// reaches the mirror field (I = 0 there, flutils.eye's early return,
+// rapt/flutils.py:97-100), the label takes a one-sided difference instead of a
+// central one: a jump of O(d / L) across the curve B(L Re - d) = B_m, which an
+// MLP smooths over.  With the split on, the query's side of that curve moves
+// the pitch-angle feature by REGIME_SHIFT, so the two smooth pieces sit apart in
+// input space; queries within 1e-5 (relative) of the curve, where the float
+// query may fall on the other side than the traced label's double one, go to
+// the fallback.
+struct SplitC {
+  int32_t on;
+  double Re, d;
+  FieldC F;
+};
+constexpr float REGIME_SHIFT = 2.5f;
+
+template <int M>
+__device__ __forceinline__ double eq_magB(const FieldC& F, double x) {
+  return magB<M>(F, 0.0, x, 0.0, 0.0);
+}
+
+// 1 for the one-sided regime; *near when the query is within 1e-5 of the curve
+__device__ int regime(const SplitC& S, float L, float adeg, bool* near) {
+  *near = false;
+  if (!S.on) return 0;
+  const double x = (double)L * S.Re;
+  double Bq, Bin;
+  switch (S.F.model) {
+    case 1: Bq = eq_magB<1>(S.F, x); Bin = eq_magB<1>(S.F, x - S.d); break;
+    case 2: Bq = eq_magB<2>(S.F, x); Bin = eq_magB<2>(S.F, x - S.d); break;
+    case 5: Bq = eq_magB<5>(S.F, x); Bin = eq_magB<5>(S.F, x - S.d); break;
+    case 8: Bq = eq_magB<8>(S.F, x); Bin = eq_magB<8>(S.F, x - S.d); break;
+    default: return 0;
+  }
+  const double sa = sin((double)adeg * (3.141592653589793 / 180.0));
+  const double Bm = Bq / (sa * sa);
+  *near = fabs(Bin - Bm) < 1e-5 * Bm;
+  return (Bin > Bm || fabs(Bin - Bm) / Bm < 1e-12) ? 1 : 0;
+}
+
+__device__ __forceinline__ void features(const float* q, float x[3], int side) {
   x[0] = (q[0] - 5.0f) * (1.0f / 3.0f);
-  x[1] = (q[1] - 49.5f) * (1.0f / 39.5f);
+  x[1] = (q[1] - 49.5f) * (1.0f / 39.5f) + (side ? REGIME_SHIFT : 0.0f);
   x[2] = (q[2] - 5.5f) * (1.0f / 1.5f);
 }
 
@@ -367,7 +411,6 @@ __global__ void k_to_bf16(const float* theta, bf16* wb, int64_t n) {
 // per wave with fragments read from L2 and 43.7 ms for 32 queries per wave
 // with LDS staging; 16 waves per block took 48.1 ms.
 constexpr int MAXM = 4;   // largest ensemble
-constexpr int IW = 8;     // waves per block (two per SIMD)
 
 struct InferArgs {
   int64_t n;
@@ -379,9 +422,10 @@ struct InferArgs {
   const float* norm;      // shift[3], scale[3]
   double mass, eV;
   float thresh;           // relative member deviation that sends a query to the fallback
+  SplitC split;           // regime of the drift label (REGIME_SHIFT)
   float* out;             // [n][3] physical outputs (T_b, |omega_d|, latitude)
   float* raw;             // optional [members][n][3] normalised outputs
-  int32_t* flag;          // [n] 1 = fallback needed
+  int32_t* flag;          // [n] in: k_regime's side | near << 1 (split on); out: 1 = fallback
 };
 
 // one member's small parameters, staged in LDS: b1, b2, (b3, w4[0..2]), b4
@@ -421,6 +465,22 @@ __global__ void k_pack1(const bf16* wb, bf16* p1) {
   p1[(int64_t)mem * 16 * 64 * 8 + idx] = v;
 }
 
+// Regime pre-pass for the inference kernel: flag[i] = side (bit 0) and
+// near (bit 1) of query i (regime()), so the double-precision field
+// evaluations stay out of the inference kernel's register budget.
+__global__ void k_regime(int64_t n, const float* query, SplitC S, int32_t* flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool near;
+  const int side = regime(S, query[3 * i], query[3 * i + 1], &near);
+  flag[i] = side | (near ? 2 : 0);
+}
+
+// IW = 8 waves per block (two per SIMD), 16 queries per wave.  The biases
+// start the accumulators.  Measured alternatives on the 2^24-query launch:
+// 2 or 4 query groups per wave (each A fragment from LDS feeding 2 or 4
+// MFMAs) spill 61-382 registers and took 51-73 ms against 44.7 ms.
+constexpr int IW = 8;
 __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
   __shared__ bf16x8 wst[PACKED / 8];
   __shared__ SmallW sw;
@@ -435,16 +495,17 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
     q[0] = valid ? A.query[qi * 3] : 5.0f;
     q[1] = valid ? A.query[qi * 3 + 1] : 45.0f;
     q[2] = valid ? A.query[qi * 3 + 2] : 5.0f;
-    features(q, xin);
+    const int rg = (valid && A.split.on) ? A.flag[qi] : 0;  // k_regime's side | near << 1
+    features(q, xin, rg & 1);
     bf16x8 xb;
 #pragma unroll
     for (int j = 0; j < 8; ++j) xb[j] = (bf16)0.0f;
     if (h == 0) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        const bf16 hi = (bf16)xin[j];
-        xb[j] = hi;
-        xb[3 + j] = (bf16)(xin[j] - (float)hi);
+        const bf16 hv = (bf16)xin[j];
+        xb[j] = hv;
+        xb[3 + j] = (bf16)(xin[j] - (float)hv);
       }
     }
     float mean[3] = {0.0f, 0.0f, 0.0f}, lo[3], hi[3];
@@ -469,24 +530,24 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
       bf16x8 act[8];
 #pragma unroll
       for (int t = 0; t < 16; ++t) {
-        f32x4 acc = {};
+        const float* bb = &sw.b1[16 * t + 4 * h];
+        f32x4 acc = {bb[0], bb[1], bb[2], bb[3]};
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P1[t * 64 + l], xb, acc, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) act[t >> 1][(t & 1) * 4 + r] =
-            (bf16)silu(acc[r] + sw.b1[16 * t + 4 * h + r]);
+        for (int r = 0; r < 4; ++r) act[t >> 1][(t & 1) * 4 + r] = (bf16)silu(acc[r]);
       }
       // layer 2
       bf16x8 nxt[8];
 #pragma unroll
       for (int t = 0; t < 16; ++t) {
-        f32x4 acc = {};
+        const float* bb = &sw.b2[16 * t + 4 * h];
+        f32x4 acc = {bb[0], bb[1], bb[2], bb[3]};
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb)
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wst[(t * 8 + kb) * 64 + l], act[kb], acc,
                                                          0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) nxt[t >> 1][(t & 1) * 4 + r] =
-            (bf16)silu(acc[r] + sw.b2[16 * t + 4 * h + r]);
+        for (int r = 0; r < 4; ++r) nxt[t >> 1][(t & 1) * 4 + r] = (bf16)silu(acc[r]);
       }
       __syncthreads();
       {
@@ -498,7 +559,9 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
       float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
 #pragma unroll 2
       for (int t = 0; t < 16; ++t) {
-        f32x4 acc = {};
+        f32x4 acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[r] = sw.l3[16 * t + 4 * h + r].x;
 #pragma unroll
         for (int kb = 0; kb < 8; ++kb)
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wst[(t * 8 + kb) * 64 + l], nxt[kb], acc,
@@ -506,7 +569,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const float4 c = sw.l3[16 * t + 4 * h + r];
-          const float v = silu(acc[r] + c.x);
+          const float v = silu(acc[r]);
           // fused multiply-adds (the library compiles without contraction)
           o0 = __builtin_fmaf(c.y, v, o0);
           o1 = __builtin_fmaf(c.z, v, o1);
@@ -531,7 +594,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
     if (valid && l < 16) {
       const Base b = baseline(q[0], q[1], q[2], A.mass, A.eV);
       const float bb[3] = {b.lnT, b.lnW, b.lnLat};
-      int bad = 0;
+      int bad = (rg & 2) ? 1 : 0;
       for (int k = 0; k < 3; ++k) {
         const float sc = A.norm[3 + k];
         const float mu = mean[k] / (float)A.members;
@@ -545,11 +608,13 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
 
 // targets: normalised log-residuals from traced labels (T_b, omega_d, lat)
 __global__ void k_targets(int64_t n, const float* query, const double* labels,
-                          const float* norm, double mass, double eV, float* X, float* T) {
+                          const float* norm, double mass, double eV, SplitC S, float* X,
+                          float* T) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float x[3];
-  features(query + 3 * i, x);
+  bool near;
+  features(query + 3 * i, x, regime(S, query[3 * i], query[3 * i + 1], &near));
   const Base b = baseline(query[3 * i], query[3 * i + 1], query[3 * i + 2], mass, eV);
   const float bb[3] = {b.lnT, b.lnW, b.lnLat};
   for (int k = 0; k < 3; ++k) {
@@ -582,6 +647,7 @@ struct rapt_mlp {
   bf16* p1 = nullptr;
   float* norm = nullptr;
   bool packed_ok = false;
+  SplitC split{};  // regime of the drift label (rapt_mlp_set_split)
   // training workspace (batch-sized)
   float *X = nullptr, *T = nullptr, *Z1 = nullptr, *Z2 = nullptr, *Z3 = nullptr;
   float *H3 = nullptr, *dH = nullptr, *Y = nullptr, *dY = nullptr;
@@ -755,6 +821,25 @@ int rapt_mlp_set_norm(rapt_mlp* m, const float* shift_scale) {
   return RAPT_OK;
 }
 
+int rapt_mlp_set_split(rapt_mlp* m, double eyegradientstep, double Re) {
+  if (!m) return RAPT_E_ARG;
+  rapt_ctx* c = m->ctx;
+  if (eyegradientstep <= 0.0) {
+    m->split.on = 0;
+    return RAPT_OK;
+  }
+  if (!(Re > 0.0)) return set_err(c, RAPT_E_ARG, "Re must be > 0");
+  if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
+  const int md = c->field.model;
+  if (md != 1 && md != 2 && md != 5 && md != 8)
+    return set_err(c, RAPT_E_ARG, "the regime split needs a dipole-type field (model %d)", md);
+  m->split.on = 1;
+  m->split.Re = Re;
+  m->split.d = eyegradientstep;
+  m->split.F = c->field;
+  return RAPT_OK;
+}
+
 int rapt_mlp_targets(rapt_mlp* m, int64_t n, const float* query, const double* labels,
                      int normalised, float* X, float* T) {
   if (!m || n <= 0 || !query || !labels || !X || !T) return m ? set_err(m->ctx, RAPT_E_ARG,
@@ -778,7 +863,7 @@ int rapt_mlp_targets(rapt_mlp* m, int64_t n, const float* query, const double* l
   } else {
     hipLaunchKernelGGL(k_targets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                        n, dq, dl, normalised ? m->norm : (const float*)nullptr, m->mass, m->eV,
-                       dX, dT);
+                       m->split, dX, dT);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(X, dX, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream) !=
             hipSuccess ||
@@ -947,6 +1032,7 @@ int rapt_mlp_infer(rapt_mlp* m, int64_t n, const float* query, float thresh, flo
     A.mass = m->mass;
     A.eV = m->eV;
     A.thresh = thresh;
+    A.split = m->split;
     A.out = dout;
     A.raw = draw;
     A.flag = dflag;
@@ -957,6 +1043,9 @@ int rapt_mlp_infer(rapt_mlp* m, int64_t n, const float* query, float thresh, flo
     int64_t blocks = (groups + IW - 1) / IW;
     if (blocks > dev_cus) blocks = dev_cus;
     HIP_OK(c, hipEventRecord(c->ev0, st));
+    if (m->split.on)
+      hipLaunchKernelGGL(k_regime, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, dq,
+                         m->split, dflag);
     hipLaunchKernelGGL(k_infer, dim3((unsigned)blocks), dim3(IW * 64), 0, st, A);
     HIP_OK(c, hipEventRecord(c->ev1, st));
     if (hipGetLastError() != hipSuccess ||

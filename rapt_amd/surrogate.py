@@ -56,7 +56,7 @@ class Surrogate:
     [members][NPARAM] (init_params by default)."""
 
     def __init__(self, field, mass=_m_pr, charge=_e, members=4, seed=0, theta0=None,
-                 device=0):
+                 device=0, split=True, params=None):
         from .constants import e as eV
         self.field = field
         self.mass, self.charge = float(mass), float(charge)
@@ -70,6 +70,14 @@ class Surrogate:
                                                C.byref(h)), "rapt_mlp_create")
         self.h = h
         self.norm = np.array([0, 0, 0, 1, 1, 1], np.float32)
+        # the two regimes of the drift label (rapt_mlp_set_split): on for the
+        # dipole-type fields whose labels the device traces
+        from .constants import Re
+        P = params or _params
+        self.split = bool(split) and field.descriptor()["model"] in (1, 2, 5, 8)
+        self.ctx.check(N.lib().rapt_mlp_set_split(
+            self.h, float(P["eyegradientstep"]) if self.split else 0.0, float(Re)),
+            "rapt_mlp_set_split")
 
     def close(self):
         if self.h:

@@ -58,7 +58,7 @@ def test_fused_inference_matches_training_forward():
     rng = np.random.default_rng(12)
     th = S.init_params(4, 3)
     th[:, 768:1024] = rng.normal(0, 0.1, (3, 256))
-    s = S.Surrogate(F.EarthDipole(), members=3, theta0=th)
+    s = S.Surrogate(F.EarthDipole(), members=3, theta0=th, split=False)
     try:
         q = _queries(1000, 13)
         from oracle import mlp_oracle as O
@@ -76,6 +76,49 @@ def test_fused_inference_matches_training_forward():
     # stay at FP32 level, the largest at a few such flips.
     assert np.median(d) <= 1e-4 * sc
     assert d.max() <= 5e-3 * sc
+
+
+def test_regime_split_features_and_flags():
+    """The drift label's two regimes (the reference's gradI falls back to a
+    one-sided difference where the inner line L Re - eyegradientstep has no
+    mirror point, rapt/flutils.py:196-227): the pitch-angle feature moves by
+    2.5 exactly for queries whose inner equatorial |B| reaches the mirror
+    field (host double-precision restatement of the test), the fused
+    inference uses the same features as the training path, and queries within
+    1e-5 of the curve are flagged for the traced fallback."""
+    import rapt_amd as rapt
+    f = F.DipolePlusUniform(Bu=(0.0, 0.0, -10e-9))
+    rng = np.random.default_rng(21)
+    n = 4000
+    q = np.c_[rng.uniform(2, 8, n), rng.uniform(60, 89, n), rng.uniform(4, 7, n)]
+    # a few queries on the curve itself: alpha from B(L Re - d) = B(L Re) / sin^2 alpha
+    d = rapt.params["eyegradientstep"]
+    for i in range(8):
+        L = 3.0 + 0.5 * i
+        Bq, Bin = f.magB([0, L * K.Re, 0, 0]), f.magB([0, L * K.Re - d, 0, 0])
+        q[i] = [L, np.degrees(np.arcsin(np.sqrt(Bq / Bin))), 5.0]
+    q32 = q.astype(np.float32)
+    Bq = np.array([f.magB([0, float(L) * K.Re, 0, 0]) for L in q32[:, 0].astype(np.float64)])
+    Bin = np.array([f.magB([0, float(L) * K.Re - d, 0, 0])
+                    for L in q32[:, 0].astype(np.float64)])
+    sa = np.sin(np.radians(q32[:, 1].astype(np.float64)))
+    Bm = Bq / sa ** 2
+    side = (Bin > Bm) | (np.abs(Bin - Bm) / Bm < 1e-12)
+    near = np.abs(Bin - Bm) < 1e-5 * Bm
+    assert 0 < side.mean() < 1 and near[:8].all()
+    s = S.Surrogate(f, members=2, seed=3)
+    try:
+        X, _ = s.targets(q32, np.ones((n, 3)))
+        base = (q32[:, 1] - 49.5) * np.float32(1 / 39.5)
+        far = ~near
+        np.testing.assert_array_equal(X[far, 1] - base[far] > 1.0, side[far])
+        Y = s.forward(X)
+        _, flag, raw, _ = s.infer(q32, thresh=1e30, raw=True)
+    finally:
+        s.close()
+    assert np.all(flag[near] == 1) and np.all(flag[far] == 0)
+    d = np.abs(raw - Y)
+    assert np.median(d) <= 1e-4 * np.abs(Y).max() and d.max() <= 5e-3 * np.abs(Y).max()
 
 
 def test_surrogate_training_and_fallback():
