@@ -44,8 +44,12 @@ enum rapt_field_model {
     RAPT_FIELD_DIPOLE_UNIFORM = 8,     /* EarthDipole(p0) + (p1, p2, p3)                */
     RAPT_FIELD_GRID = 9,               /* B, E sampled on a grid (rapt_grid_set)
                                                                    rapt/fields.py:513-814 */
-    RAPT_FIELD_NMODELS = 10
+    RAPT_FIELD_EXPR = 10,              /* a user-written field (rapt_field_expr_set)
+                                                                   rapt/fields.py:5-74 */
+    RAPT_FIELD_NMODELS = 11
 };
+#define RAPT_EXPR_MAXCODE 256
+#define RAPT_EXPR_MAXCONST 128
 
 /* Field descriptor: replaces the _Field plugin object (rapt/fields.py:5-41). */
 typedef struct {
@@ -172,6 +176,27 @@ int rapt_field_set(rapt_ctx *ctx, const rapt_field_desc *field);
 int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt, const double *x,
                   const double *y, const double *z, const double *t, const double *B,
                   const double *E);
+
+/* A user-written field for RAPT_FIELD_EXPR: replaces a _Field subclass that
+ * overrides B and/or E in Python (rapt/fields.py:5-74, the contract of the
+ * reference's "Creating new fields" example), whose methods the reference's
+ * integrator calls per right-hand side.  The Python front end traces those
+ * methods into register programs (rapt_amd/fieldexpr.py); the device
+ * interprets them in FP64.  code: (nb + ne) x 4 int32 (opcode, destination,
+ * operand a, operand b) over 32 registers, 0-3 = t, x, y, z; each program
+ * leaves its vector in registers 4-6; B's program is the first nb
+ * instructions, E's the next ne (ne = 0: E = 0).  Opcodes: 0 register[d] =
+ * consts[a], 1 a+b, 2 a-b, 3 a*b, 4 a/b, 5 -a, 6 pow(a,b), 7 sqrt, 8 exp,
+ * 9 log, 10 sin, 11 cos, 12 tan, 13 tanh, 14 sinh, 15 cosh, 16 atan, 17 asin,
+ * 18 acos, 19 fabs, 20 atan2(a,b), 21 a (move), 22 log10, 23 hypot(a,b).
+ * At most RAPT_EXPR_MAXCODE instructions and RAPT_EXPR_MAXCONST constants;
+ * the library copies both into device memory owned by the context.  Then
+ * rapt_field_set with model RAPT_FIELD_EXPR (params unused) selects it.
+ * Bounce-path entry points (rapt_halfbounce, rapt_eye, rapt_bounce_*) do not
+ * take it; FP32-field precision does not either; GuidingCenter runs use one
+ * lane per particle whatever lanes_per_particle asks (results are the same). */
+int rapt_field_expr_set(rapt_ctx *ctx, int32_t nb, int32_t ne, const int32_t *code,
+                        int32_t nconst, const double *consts);
 
 /* Batched field evaluation on the device (the _Field methods,
  * rapt/fields.py:76-280).  what: 0 B, 1 E, 2 unitb, 3 magB, 4 gradB,

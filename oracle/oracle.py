@@ -15,6 +15,7 @@ _LIB_PATH = os.environ.get("RAPT_ORACLE_LIB") or os.path.join(_HERE, "librapt_or
 
 ZERO, EARTH_DIPOLE, DOUBLE_DIPOLE, UNIFORM_BZ, UNIFORM_CROSSED_EB, VAR_EARTH_DIPOLE, \
     PARABOLIC, CURRENT_SHEET, DIPOLE_UNIFORM = range(9)
+EXPR = 10
 EVAL = dict(B=0, E=1, unitb=2, magB=3, gradB=4, jacobianB=5, curvature=6, curlb=7,
             dBdt=8, dbdt=9, lengthscale=10, timescale=11)
 EOM = dict(TaoChanBrizardEOM=0, BrizardChanEOM=1, NorthropTellerEOM=2)
@@ -23,7 +24,8 @@ DOP853, DOPRI5, RKF45 = 0, 1, 2
 
 class Field(C.Structure):
     _fields_ = [("model", C.c_int32), ("is_static", C.c_int32), ("gstep", C.c_double),
-                ("tstep", C.c_double), ("p", C.c_double * 16)]
+                ("tstep", C.c_double), ("p", C.c_double * 16), ("code", C.c_void_p),
+                ("consts", C.c_void_p), ("nb", C.c_int32), ("ne", C.c_int32)]
 
 
 class Params(C.Structure):
@@ -142,9 +144,21 @@ def make_field(model, params=(), is_static=True, gstep=1e-6, tstep=1e-3):
 
 
 def field_from_desc(desc):
-    """desc: dict with model, params, static, gstep, tstep (the product's field descriptor)."""
-    return make_field(desc["model"], desc["params"], desc["static"], desc["gstep"],
-                      desc["tstep"])
+    """desc: dict with model, params, static, gstep, tstep (the product's field
+    descriptor), and for a user-written field its traced program ``expr``
+    (code [n][4] int32, consts, nb, ne)."""
+    f = make_field(desc["model"], desc["params"], desc["static"], desc["gstep"],
+                   desc["tstep"])
+    prog = desc.get("expr")
+    if prog is not None:
+        code = np.ascontiguousarray(prog.code, dtype=np.int32)
+        consts = np.ascontiguousarray(prog.consts, dtype=np.float64)
+        if consts.size == 0:
+            consts = np.zeros(1)
+        f._keep = (code, consts)  # the struct holds raw pointers into these
+        f.code, f.consts = code.ctypes.data, consts.ctypes.data
+        f.nb, f.ne = prog.nb, prog.ne
+    return f
 
 
 def make_params(rtol=1.49012e-8, atol=1.49012e-8, cyclotronresolution=10, bounceresolution=10,

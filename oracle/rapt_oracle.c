@@ -40,6 +40,50 @@ static inline double sq(double x) { return pow(x, 2.0); }
 /* ======================================================================
  * Field models: rapt/fields.py
  * ==================================================================== */
+/* A user-written field (a _Field subclass overriding B/E,
+ * rapt/fields.py:5-74): the operations its Python code performs on the
+ * scalars of tpos, in order, as the reference evaluates them (numpy float64
+ * scalar arithmetic; math functions of the C library). */
+static void expr_run(const orc_field *f, int lo, int n, const double *tp, double *out) {
+    double r[32];
+    r[0] = tp[0]; r[1] = tp[1]; r[2] = tp[2]; r[3] = tp[3];
+    r[4] = r[5] = r[6] = 0.0;
+    for (int i = lo; i < lo + n; ++i) {
+        const int32_t *c = f->code + 4 * i;
+        const double a = c[0] == 0 ? 0.0 : r[c[2] & 31], b = r[c[3] & 31];
+        double v;
+        switch (c[0]) {
+        case 0: v = f->consts[c[2]]; break;
+        case 1: v = a + b; break;
+        case 2: v = a - b; break;
+        case 3: v = a * b; break;
+        case 4: v = a / b; break;
+        case 5: v = -a; break;
+        case 6: v = pow(a, b); break;
+        case 7: v = sqrt(a); break;
+        case 8: v = exp(a); break;
+        case 9: v = log(a); break;
+        case 10: v = sin(a); break;
+        case 11: v = cos(a); break;
+        case 12: v = tan(a); break;
+        case 13: v = tanh(a); break;
+        case 14: v = sinh(a); break;
+        case 15: v = cosh(a); break;
+        case 16: v = atan(a); break;
+        case 17: v = asin(a); break;
+        case 18: v = acos(a); break;
+        case 19: v = fabs(a); break;
+        case 20: v = atan2(a, b); break;
+        case 21: v = a; break;
+        case 22: v = log10(a); break;
+        case 23: v = hypot(a, b); break;
+        default: v = NAN; break;
+        }
+        r[c[1] & 31] = v;
+    }
+    out[0] = r[4]; out[1] = r[5]; out[2] = r[6];
+}
+
 static void field_B(const orc_field *f, const double *tp, double *B) {
     const double t = tp[0];
     double x = tp[1], y = tp[2], z = tp[3];
@@ -97,6 +141,9 @@ static void field_B(const orc_field *f, const double *tp, double *B) {
         B[0] = f->p[0] * tanh(z / f->p[1]);
         B[1] = 0.0; B[2] = f->p[2];
         return;
+    case ORC_FIELD_EXPR:
+        expr_run(f, 0, f->nb, tp, B);
+        return;
     default:
         B[0] = B[1] = B[2] = 0.0;
         return;
@@ -104,9 +151,10 @@ static void field_B(const orc_field *f, const double *tp, double *B) {
 }
 
 static void field_E(const orc_field *f, const double *tp, double *E) {
-    (void)tp;
     E[0] = E[1] = E[2] = 0.0;
-    if (f->model == ORC_FIELD_UNIFORM_CROSSED_EB) { /* rapt/fields.py:413-427 */
+    if (f->model == ORC_FIELD_EXPR) {
+        if (f->ne > 0) expr_run(f, f->nb, f->ne, tp, E);
+    } else if (f->model == ORC_FIELD_UNIFORM_CROSSED_EB) { /* rapt/fields.py:413-427 */
         E[1] = f->p[1];
     } else if (f->model == ORC_FIELD_CURRENT_SHEET) {
         E[0] = f->p[3]; E[1] = f->p[4]; E[2] = f->p[5];

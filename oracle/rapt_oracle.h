@@ -34,7 +34,8 @@ enum {
     ORC_FIELD_VAR_EARTH_DIPOLE = 5,  /* p0 = -B0*Re**3, p1 = amp, p2 = period          */
     ORC_FIELD_PARABOLIC = 6,         /* p0 = B0, p1 = Bn, p2 = d, p3 = module B0        */
     ORC_FIELD_CURRENT_SHEET = 7,     /* p0 = B0, p1 = lambda, p2 = Bn, p3..5 = E        */
-    ORC_FIELD_DIPOLE_UNIFORM = 8     /* p0 = coeff (-3 B0 Re^3), p1..3 = uniform B      */
+    ORC_FIELD_DIPOLE_UNIFORM = 8,    /* p0 = coeff (-3 B0 Re^3), p1..3 = uniform B      */
+    ORC_FIELD_EXPR = 10              /* user-written B/E as register programs (code)    */
 };
 
 typedef struct {
@@ -43,6 +44,12 @@ typedef struct {
     double gstep;   /* gradientstepsize  */
     double tstep;   /* timederivstepsize */
     double p[16];
+    /* ORC_FIELD_EXPR: instructions (opcode, dest, a, b) over 32 registers,
+     * 0-3 = t, x, y, z, results in 4-6; B's program is code[0, nb), E's
+     * code[nb, nb + ne) (ne = 0: E = 0) */
+    const int32_t *code;
+    const double *consts;
+    int32_t nb, ne;
 } orc_field;
 
 /* what= selector for orc_field_eval */

@@ -115,6 +115,7 @@ def lib():
     L.rapt_batch_set_cost.argtypes = [vp, dp]
     L.rapt_grid_set.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp, dp, dp,
                                 dp, dp]
+    L.rapt_field_expr_set.argtypes = [vp, C.c_int32, C.c_int32, i32p, C.c_int32, dp]
     L.rapt_eye.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
     L.rapt_particle_gc.argtypes = [vp, C.c_int64, dp, dp, dp, C.c_int32, C.c_double, C.c_int32,
                                    dp, i32p]
@@ -134,7 +135,7 @@ def lib():
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
                "rapt_batch_reset", "rapt_batch_set_cost", "rapt_eye", "rapt_particle_gc",
-               "rapt_grid_set", "rapt_trace_device_noalloc"):
+               "rapt_grid_set", "rapt_field_expr_set", "rapt_trace_device_noalloc"):
         getattr(L, nm).restype = C.c_int
     _lib = L
     return L
@@ -166,6 +167,7 @@ class Context:
         self.device = device
         self._field_key = None
         self._grid_id = 0
+        self._expr_key = None
 
     def check(self, rc, what):
         if rc != 0:
@@ -174,10 +176,19 @@ class Context:
 
     def set_field(self, desc):
         grid = desc.get("grid")
+        prog = desc.get("expr")
         key = (desc["model"], desc["static"], desc["gstep"], desc["tstep"], desc["params"],
-               grid._grid_id if grid is not None else 0)
+               grid._grid_id if grid is not None else 0,
+               prog.key() if prog is not None else None)
         if key == self._field_key:
             return
+        if prog is not None and prog.key() != self._expr_key:
+            code = np.ascontiguousarray(prog.code, dtype=np.int32)
+            consts = np.ascontiguousarray(prog.consts, dtype=np.float64)
+            self.check(lib().rapt_field_expr_set(
+                self.h, prog.nb, prog.ne, code.ctypes.data_as(C.POINTER(C.c_int32)),
+                len(consts), _dp(consts) if len(consts) else None), "rapt_field_expr_set")
+            self._expr_key = prog.key()
         if grid is not None and self._grid_id != grid._grid_id:
             x, y, z, t, B, E = (np.ascontiguousarray(a, dtype=np.float64)
                                 for a in grid.grid_arrays())

@@ -144,6 +144,7 @@ extern "C" int rapt_particle_gc(rapt_ctx* c, int64_t n, const double* rows, cons
       case 7: launch_pgc<7>(c->stream, A); break;
       case 8: launch_pgc<8>(c->stream, A); break;
       case 9: launch_pgc<9>(c->stream, A); break;
+      case 10: launch_pgc<10>(c->stream, A); break;
     }
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(out, d_out, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, c->stream) !=

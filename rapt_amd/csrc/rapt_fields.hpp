@@ -31,8 +31,24 @@ constexpr double TWO_PI = 6.283185307179586;
 enum FieldModel : int {
   F_ZERO = 0, F_EARTH_DIPOLE = 1, F_DOUBLE_DIPOLE = 2, F_UNIFORM_BZ = 3,
   F_UNIFORM_CROSSED_EB = 4, F_VAR_EARTH_DIPOLE = 5, F_PARABOLIC = 6,
-  F_CURRENT_SHEET = 7, F_DIPOLE_UNIFORM = 8, F_GRID = 9
+  F_CURRENT_SHEET = 7, F_DIPOLE_UNIFORM = 8, F_GRID = 9, F_EXPR = 10
 };
+
+// A user-written field (a _Field subclass overriding B and/or E,
+// rapt/fields.py:5-74) traced into register programs by the Python front end:
+// instruction i is code[4i..4i+3] = (opcode, destination, operand a, operand
+// b) over 32 double registers, 0-3 holding t, x, y, z; a program leaves its
+// three components in registers 4-6.  B's program is the first nb
+// instructions, E's the next ne (ne = 0: E = 0).  Opcodes: 0 const (a =
+// constant index), 1 add, 2 sub, 3 mul, 4 div, 5 neg, 6 pow, 7 sqrt, 8 exp,
+// 9 log, 10 sin, 11 cos, 12 tan, 13 tanh, 14 sinh, 15 cosh, 16 atan, 17 asin,
+// 18 acos, 19 fabs, 20 atan2, 21 mov, 22 log10, 23 hypot.
+struct ExprC {
+  const int32_t* code;
+  const double* k;
+  int32_t nb, ne;
+};
+constexpr int EXPR_NREG = 32;
 
 // A gridded field (Grid, rapt/fields.py:513-814) in HBM: B and E sampled on
 // a rectilinear grid (axes need not be uniform) at nt >= 1 times.  Values
@@ -63,7 +79,10 @@ struct FieldC {
   // values the device would compute)
   double i2d;   // 1 / (2 gstep): central-difference scale in space
   double ihts;  // 0.5 / tstep: central-difference scale in time (0 for static fields)
-  GridC grid;   // F_GRID only
+  union {
+    GridC grid;   // F_GRID only
+    ExprC expr;   // F_EXPR only
+  };
 };
 
 template <class T>
@@ -239,6 +258,58 @@ __device__ __forceinline__ V3 grid_interp(const GridC& G, const double* V, doubl
   return V3{lerp(a.x, b.x, ft), lerp(a.y, b.y, ft), lerp(a.z, b.z, ft)};
 }
 
+// ---- user-written fields ------------------------------------------------------
+// One program on the registers (t, x, y, z, ...); every lane runs the same
+// instruction stream, so the instruction words are uniform.  Out of line (one
+// copy per kernel), taking the program by value: a reference into the
+// kernel's argument block would make the caller copy the whole block to
+// scratch to pass its address.  The register file is per-lane scratch (256
+// bytes), indexed by the instruction words.
+__device__ __noinline__ V3 expr_eval(ExprC X, int lo, int n, double t, double x, double y,
+                                     double z) {
+  double r[EXPR_NREG];
+  r[0] = t;
+  r[1] = x;
+  r[2] = y;
+  r[3] = z;
+  r[4] = r[5] = r[6] = 0.0;
+  for (int i = lo; i < lo + n; ++i) {
+    const int32_t op = X.code[4 * i], d = X.code[4 * i + 1], ia = X.code[4 * i + 2],
+                  ib = X.code[4 * i + 3];
+    const double a = r[ia & (EXPR_NREG - 1)], b = r[ib & (EXPR_NREG - 1)];
+    double v;
+    switch (op) {
+      case 0: v = X.k[ia]; break;
+      case 1: v = a + b; break;
+      case 2: v = a - b; break;
+      case 3: v = a * b; break;
+      case 4: v = a / b; break;
+      case 5: v = -a; break;
+      case 6: v = pow(a, b); break;
+      case 7: v = sqrt(a); break;
+      case 8: v = exp(a); break;
+      case 9: v = log(a); break;
+      case 10: v = sin(a); break;
+      case 11: v = cos(a); break;
+      case 12: v = tan(a); break;
+      case 13: v = tanh(a); break;
+      case 14: v = sinh(a); break;
+      case 15: v = cosh(a); break;
+      case 16: v = atan(a); break;
+      case 17: v = asin(a); break;
+      case 18: v = acos(a); break;
+      case 19: v = fabs(a); break;
+      case 20: v = atan2(a, b); break;
+      case 21: v = a; break;
+      case 22: v = log10(a); break;
+      case 23: v = hypot(a, b); break;
+      default: v = __builtin_nan(""); break;
+    }
+    r[d & (EXPR_NREG - 1)] = v;
+  }
+  return V3{r[4], r[5], r[6]};
+}
+
 // ---- B(t, r) ---------------------------------------------------------------
 // T = double is the tracer's arithmetic; T = float serves the FP32-field
 // precision mode (constants rounded once to float).
@@ -303,6 +374,9 @@ __device__ __forceinline__ V3T<T> fieldB(const FieldC& F, T t, T x, T y, T z) {
   } else if constexpr (M == F_GRID) {
     static_assert(sizeof(T) == sizeof(double), "gridded fields are FP64");
     return grid_interp(F.grid, F.grid.B, t, x, y, z);
+  } else if constexpr (M == F_EXPR) {
+    static_assert(sizeof(T) == sizeof(double), "user-written fields are FP64");
+    return expr_eval(F.expr, 0, F.expr.nb, t, x, y, z);
   } else {
     return V{T(0.0), T(0.0), T(0.0)};
   }
@@ -315,6 +389,10 @@ __device__ __forceinline__ V3T<T> fieldE(const FieldC& F, T t, T x, T y, T z) {
     static_assert(sizeof(T) == sizeof(double), "gridded fields are FP64");
     if (!F.grid.E) return V3T<T>{T(0.0), T(0.0), T(0.0)};
     return grid_interp(F.grid, F.grid.E, t, x, y, z);
+  } else if constexpr (M == F_EXPR) {
+    static_assert(sizeof(T) == sizeof(double), "user-written fields are FP64");
+    if (F.expr.ne == 0) return V3T<T>{T(0.0), T(0.0), T(0.0)};
+    return expr_eval(F.expr, F.expr.nb, F.expr.ne, t, x, y, z);
   } else if constexpr (M == F_UNIFORM_CROSSED_EB) {
     return V3T<T>{T(0.0), T(F.p[1]), T(0.0)};
   } else if constexpr (M == F_CURRENT_SHEET) {
@@ -326,7 +404,7 @@ __device__ __forceinline__ V3T<T> fieldE(const FieldC& F, T t, T x, T y, T z) {
 
 template <int M>
 constexpr bool has_E() {
-  return M == F_UNIFORM_CROSSED_EB || M == F_CURRENT_SHEET || M == F_GRID;
+  return M == F_UNIFORM_CROSSED_EB || M == F_CURRENT_SHEET || M == F_GRID || M == F_EXPR;
 }
 
 template <int M>

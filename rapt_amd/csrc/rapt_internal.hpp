@@ -61,6 +61,9 @@ struct rapt_ctx {
   double* grid_mem = nullptr;    // axes and point-layout values
   double* grid_cells = nullptr;  // cell-layout values (nullptr: point layout)
   bool grid_set = false;
+  rapt::ExprC expr{};            // uploaded by rapt_field_expr_set (device memory below)
+  void* expr_mem = nullptr;
+  bool expr_set = false;
   // device address of this device's allocation-free scratch (the work-queue
   // and step counters of rapt_trace_device_noalloc), from hipGetSymbolAddress
   unsigned long long* noalloc_scratch = nullptr;

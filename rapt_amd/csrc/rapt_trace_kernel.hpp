@@ -82,10 +82,11 @@ struct GcRHS {
 // and Particle kernels and hides the dependent-chain latency better (one-lane
 // GC +21% FP64, +31% FP32-field on 1,048,576 particles); one wave for
 // gridded fields, where two spill 180 bytes per lane in the Particle kernel
-// and ran 15% slower (65,536 protons in a 257 x 257 x 193 dipole grid).
+// and ran 15% slower (65,536 protons in a 257 x 257 x 193 dipole grid), and
+// for user-written fields, whose interpreter calls carry a register file.
 template <int M>
 constexpr int trace_waves() {
-  return M == F_GRID ? 1 : 2;
+  return M == F_GRID || M == F_EXPR ? 1 : 2;
 }
 template <int M, int KIND, int P, bool SP = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(trace_waves<M>())))

@@ -208,6 +208,7 @@ static void launch_model(rapt_ctx* c, const KArgs& A, int block, int64_t lanes) 
     case 7: launch_trace<7, K>(c, A, block, lanes); break;
     case 8: launch_trace<8, K>(c, A, block, lanes); break;
     case 9: launch_trace<9, K>(c, A, block, lanes); break;
+    case 10: launch_trace<10, K>(c, A, block, lanes); break;
   }
 }
 
@@ -250,6 +251,7 @@ int rapt_ctx_destroy(rapt_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   (void)hipFree(c->grid_mem);
   (void)hipFree(c->grid_cells);
+  (void)hipFree(c->expr_mem);
   delete c;
   return RAPT_OK;
 }
@@ -265,12 +267,56 @@ int rapt_field_set(rapt_ctx* c, const rapt_field_desc* f) {
     return set_err(c, RAPT_E_ARG, "timederivstepsize must be > 0");
   if (f->model == RAPT_FIELD_GRID && !c->grid_set)
     return set_err(c, RAPT_E_STATE, "gridded field: no grid uploaded (rapt_grid_set)");
+  if (f->model == RAPT_FIELD_EXPR && !c->expr_set)
+    return set_err(c, RAPT_E_STATE, "user-written field: no program uploaded "
+                                    "(rapt_field_expr_set)");
   static_assert(offsetof(FieldC, i2d) == sizeof(rapt_field_desc), "descriptor layout");
   memcpy(&c->field, f, sizeof(rapt_field_desc));
   c->field.i2d = 1.0 / (2.0 * f->gstep);
   c->field.ihts = f->is_static ? 0.0 : 0.5 / f->tstep;
-  c->field.grid = f->model == RAPT_FIELD_GRID ? c->grid : GridC{};
+  if (f->model == RAPT_FIELD_EXPR)
+    c->field.expr = c->expr;
+  else
+    c->field.grid = f->model == RAPT_FIELD_GRID ? c->grid : GridC{};
   c->field_set = true;
+  return RAPT_OK;
+}
+
+int rapt_field_expr_set(rapt_ctx* c, int32_t nb, int32_t ne, const int32_t* code, int32_t nconst,
+                        const double* consts) {
+  if (!c || !code || (nconst > 0 && !consts)) return RAPT_E_ARG;
+  if (nb < 1 || ne < 0 || nb + ne > RAPT_EXPR_MAXCODE || nconst < 0 ||
+      nconst > RAPT_EXPR_MAXCONST)
+    return set_err(c, RAPT_E_ARG, "field program: %d + %d instructions, %d constants (at most "
+                   "%d instructions, %d constants)", nb, ne, nconst, RAPT_EXPR_MAXCODE,
+                   RAPT_EXPR_MAXCONST);
+  for (int32_t i = 0; i < nb + ne; ++i) {
+    const int32_t op = code[4 * i], d = code[4 * i + 1], a = code[4 * i + 2], b = code[4 * i + 3];
+    const bool regs = d >= 4 && d < EXPR_NREG && a >= 0 && b >= 0 &&
+                      (op == 0 ? a < nconst : a < EXPR_NREG) && b < EXPR_NREG;
+    if (op < 0 || op > 23 || !regs)
+      return set_err(c, RAPT_E_ARG, "field program: bad instruction %d (%d %d %d %d)", i, op, d,
+                     a, b);
+  }
+  HIP_OK(c, hipSetDevice(c->device));
+  const size_t kb = sizeof(double) * (size_t)(nconst > 0 ? nconst : 1);
+  const size_t cb = sizeof(int32_t) * 4 * (size_t)(nb + ne);
+  void* mem = nullptr;
+  if (hipMalloc(&mem, kb + cb) != hipSuccess)
+    return set_err(c, RAPT_E_OOM, "field program allocation failed");
+  if ((nconst > 0 && hipMemcpy(mem, consts, sizeof(double) * nconst, hipMemcpyHostToDevice) !=
+                         hipSuccess) ||
+      hipMemcpy((char*)mem + kb, code, cb, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(mem);
+    return set_err(c, RAPT_E_HIP, "field program upload failed");
+  }
+  // work still queued on the stream may read the previous program
+  HIP_OK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(c->expr_mem);
+  c->expr_mem = mem;
+  c->expr = ExprC{(const int32_t*)((char*)mem + kb), (const double*)mem, nb, ne};
+  c->expr_set = true;
+  if (c->field_set && c->field.model == RAPT_FIELD_EXPR) c->field.expr = c->expr;
   return RAPT_OK;
 }
 
@@ -406,6 +452,7 @@ int rapt_field_eval(rapt_ctx* c, int what, int64_t n, const double* tpos, double
       case 7: launch_field_eval<7>(c, what, n, dt, dout, w); break;
       case 8: launch_field_eval<8>(c, what, n, dt, dout, w); break;
       case 9: launch_field_eval<9>(c, what, n, dt, dout, w); break;
+      case 10: launch_field_eval<10>(c, what, n, dt, dout, w); break;
     }
     if (hipGetLastError() != hipSuccess) {
       rc = set_err(c, RAPT_E_HIP, "field kernel launch failed");
@@ -591,6 +638,8 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
     return set_err(c, RAPT_E_ARG, "precision must be RAPT_PREC_FP64 or RAPT_PREC_FP32_FIELD");
   if (d->precision == RAPT_PREC_FP32_FIELD && c->field.model == RAPT_FIELD_GRID)
     return set_err(c, RAPT_E_ARG, "gridded fields are traced in FP64 only");
+  if (d->precision == RAPT_PREC_FP32_FIELD && c->field.model == RAPT_FIELD_EXPR)
+    return set_err(c, RAPT_E_ARG, "user-written fields are traced in FP64 only");
   if (d->mode == RAPT_MODE_ADAPTIVE && !(d->gctimestep > 0.0))
     return set_err(c, RAPT_E_ARG, "ADAPTIVE needs params['GCtimestep'] > 0");
   if (d->mode == RAPT_MODE_PARTICLE && !(d->cyclotronresolution > 0.0))
@@ -675,9 +724,13 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
   if (lanes > b->n) lanes = b->n;
   if (lanes < 1) lanes = 1;
-  // the grouped kernels (two, four or eight lanes, mixed) are FP64 only
+  // the grouped kernels (two, four or eight lanes, mixed) are FP64 only; a
+  // user-written field runs one lane per particle (the grouped kernels keep
+  // too many registers live across the interpreter's calls: 1-1.2 KB of
+  // spills per lane)
   const bool grouped = d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOPRI5 &&
-                    d->lanes_per_particle != 1 && d->precision == RAPT_PREC_FP64;
+                       d->lanes_per_particle != 1 && d->precision == RAPT_PREC_FP64 &&
+                       c->field.model != RAPT_FIELD_EXPR;
   // Default block: 256 lanes, except for launches of fewer than four waves
   // per CU, which take 64 so that the waves spread over every CU (a CU's
   // waves share its scalar unit and instruction fetch): 1,024 Particle-mode

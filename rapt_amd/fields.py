@@ -6,8 +6,12 @@ Each model keeps the reference's host methods (``B``, ``E``, ``unitb``,
 so user code that probes a field keeps working.  Tracing never calls them:
 ``advance()`` hands the model's ``descriptor()`` (model id + constants) to the
 HIP library, whose kernels evaluate the same formulas on the device.  A user
-subclass that only overrides ``B``/``E`` in Python has no descriptor and
-cannot be traced on the GPU: ``descriptor()`` raises NotImplementedError.
+subclass that overrides ``B``/``E`` in Python (the reference's way to define
+a new field) is traced: ``descriptor()`` runs the overrides once on symbolic
+coordinates and hands the device the resulting register program
+(``fieldexpr``; model ``FIELD_EXPR``).  Overrides that branch on field values
+or leave numpy's elementwise math cannot be traced and raise
+NotImplementedError.
 
 Reference: rapt/fields.py:5-814.  ``Grid`` (rapt/fields.py:513-814) keeps
 every parsed snapshot resident in device memory and is interpolated on the
@@ -17,6 +21,7 @@ import itertools
 
 import numpy as np
 
+from . import fieldexpr
 from .constants import B0, Re
 
 # model ids shared with include/rapt_hip.h (enum rapt_field_model)
@@ -30,6 +35,7 @@ FIELD_PARABOLIC = 6
 FIELD_CURRENT_SHEET = 7
 FIELD_DIPOLE_UNIFORM = 8
 FIELD_GRID = 9
+FIELD_EXPR = 10
 
 
 class _Field:
@@ -53,18 +59,28 @@ class _Field:
     def descriptor(self):
         """Model id and constants for the HIP kernels.
 
-        Raises NotImplementedError for a model the device cannot evaluate
-        (for instance a Python subclass overriding ``B`` without a model id).
+        A built-in model gives its id and constants.  A subclass whose ``B``
+        or ``E`` is Python code of its own (the reference's plugin contract,
+        rapt/fields.py:5-74) is traced into a register program at every call,
+        so attributes changed since the last call take effect as they would
+        in the reference: model ``FIELD_EXPR`` with the program under
+        ``"expr"``.  Raises NotImplementedError when the overrides cannot be
+        traced.
         """
-        if self._model is None or type(self).B is not _MODEL_B.get(self._model, type(self).B):
-            raise NotImplementedError(
-                "%s has no device field model; the GPU tracer evaluates only the built-in "
-                "analytic models" % type(self).__name__)
-        if type(self).E is not _MODEL_E.get(self._model, type(self).E):
-            raise NotImplementedError("%s overrides E(); no device model" % type(self).__name__)
+        common = dict(static=bool(self.static), gstep=float(self.gradientstepsize),
+                      tstep=float(self.timederivstepsize))
+        builtin = (self._model is not None and
+                   type(self).B is _MODEL_B.get(self._model, type(self).B) and
+                   type(self).E is _MODEL_E.get(self._model, type(self).E))
+        if not builtin:
+            try:
+                prog = fieldexpr.trace(self)
+            except NotImplementedError as e:
+                raise NotImplementedError("%s cannot run on the device: %s"
+                                          % (type(self).__name__, e)) from None
+            return dict(model=FIELD_EXPR, params=(), expr=prog, **common)
         return dict(model=self._model, params=tuple(float(v) for v in self._params()),
-                    static=bool(self.static), gstep=float(self.gradientstepsize),
-                    tstep=float(self.timederivstepsize))
+                    **common)
 
     # ---- host API (rapt/fields.py:43-280) -----------------------------------
     def B(self, tpos):

@@ -29,3 +29,18 @@ def oracle():
     from oracle import oracle as O
     O.lib()
     return O
+
+
+@pytest.fixture(scope="session")
+def user_fields():
+    """User-written field classes (tests/golden/userfields.py) built on
+    rapt_amd.fields, the same class bodies the golden fixtures ran under the
+    reference."""
+    import importlib.util
+
+    import rapt_amd.fields as F
+    spec = importlib.util.spec_from_file_location("userfields",
+                                                  os.path.join(GOLDEN, "userfields.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.classes(F)

@@ -35,6 +35,9 @@ from rapt.rkf import rkf as ref_rkf  # noqa: E402
 
 from rapt_amd import workloads as W  # noqa: E402  (seeded inputs only)
 
+sys.path.insert(0, HERE)
+import userfields  # noqa: E402
+
 Re = rapt.Re
 DEFAULT_PARAMS = dict(rapt.params)
 
@@ -45,32 +48,10 @@ def reset_params(**kw):
     rapt.params.update(kw)
 
 
-class RefCurrentSheet(rf._Field):
-    """B = (B0 tanh(z/lam), 0, Bn), E uniform: a user field written the
-    reference's way (override B and E, set static and gradientstepsize)."""
-
-    def __init__(self, B0=20e-9, lam=0.5 * Re, Bn=1e-9, E=(0.0, 0.3e-3, 0.0)):
-        rf._Field.__init__(self)
-        self.B0, self.lam, self.Bn = B0, lam, Bn
-        self._E = np.array(E, dtype=float)
-        self.static = not np.any(self._E != 0)
-        self.gradientstepsize = lam / 1000
-
-    def B(self, tpos):
-        z = tpos[3]
-        return np.array([self.B0 * np.tanh(z / self.lam), 0, self.Bn])
-
-    def E(self, tpos):
-        return self._E.copy()
-
-
-class RefDipolePlusUniform(rf.EarthDipole):
-    def __init__(self, Bu=(0.0, 0.0, -10e-9)):
-        rf.EarthDipole.__init__(self)
-        self._Bu = np.array(Bu, dtype=float)
-
-    def B(self, tpos):
-        return rf.EarthDipole.B(self, tpos) + self._Bu
+# user-written fields (subclasses overriding B/E), shared with the tests
+_USER = userfields.classes(rf)
+RefCurrentSheet = _USER["SheetField"]
+RefDipolePlusUniform = _USER["DipolePlusUniformField"]
 
 
 def field_cases():
@@ -426,8 +407,74 @@ def make_bounce():
     np.savez_compressed(os.path.join(HERE, "bounce.npz"), **out)
 
 
+def make_userfields():
+    """The reference's tracers in user-written fields (tests/golden/userfields.py):
+    the fields' host methods at sample points, Particle and GuidingCenter
+    trajectories.  GuidingCenter runs set GCtimestep (the default output step
+    comes from a field-line trace)."""
+    out = {}
+    U = _USER
+    what = ["B", "E", "magB", "gradB", "curvature", "dBdt", "lengthscale"]
+    cases = [("ChargedDipole", U["ChargedDipole"](), 1.0),
+             ("TwinDipole", U["TwinDipole"](), Re),
+             ("BreathingSheet", U["BreathingSheet"](), Re)]
+    for name, f, scale in cases:
+        kind = "sheet" if name == "BreathingSheet" else "dipole"
+        pts = sample_points(kind, 24, 12)
+        pts[:, 1:] *= scale / Re
+        out[name + "/tpos"] = pts
+        for w in what:
+            if w == "dBdt" and f.static:
+                continue
+            vals = [np.array(getattr(f, w)(np.array(tp, dtype=float)), dtype=float).reshape(-1)
+                    for tp in pts]
+            out[name + "/" + w] = np.array(vals)
+    # the reference's new-field example: a 1 eV proton at 5 m in the charged dipole
+    reset_params()
+    f = U["ChargedDipole"](Q=1e-6)
+    p = rapt.Particle([5.0, 0.0, 0.0], [0.0, 1.0, 0.0], 0, rapt.m_pr, rapt.e, f)
+    # p.setke(1): its re-initialisation compares an array with [] and raises
+    # under numpy 2, so its arithmetic (rapt/Particle.py:168-187) is done here
+    s = ru.speedfromKE(1, p.mass, "ev")
+    mom = p.trajectory[-1, 4:]
+    v = mom / np.sqrt(p.mass ** 2 + np.dot(mom, mom) / rapt.c ** 2)
+    v = v * (s / np.sqrt(np.dot(v, v)))
+    p = rapt.Particle([5.0, 0.0, 0.0], [float(a) for a in v], 0, rapt.m_pr, rapt.e, f)
+    out["cd_particle/row0"] = p.trajectory[0].copy()
+    p.advance(4e-4)
+    out["cd_particle/traj"] = p.trajectory.copy()
+    out["cd_particle/tcur"] = np.array([p.tcur])
+    cs = W.current_sheet_protons(n=2)
+    dp = W.dipole_protons_1mev(n=2)
+    runs = [("twin_particle", U["TwinDipole"](), dp["pos"][1], dp["vel"][1], rapt.m_pr,
+             rapt.e, 2.0),
+            ("breathing_particle", U["BreathingSheet"](), cs["pos"][1], cs["vel"][1],
+             rapt.m_pr, rapt.e, 10.0)]
+    for name, f, pos, vel, m, q, delta in runs:
+        tr, tc = run_particle(f, pos, vel, m, q, delta)
+        out[name + "/in"] = np.r_[pos, vel, m, q, delta]
+        out[name + "/traj"] = tr
+        out[name + "/tcur"] = np.array([tc])
+    wl = W.doubledipole_electrons_gc(n=2)
+    gcs = [("cd_gc", U["ChargedDipole"](Q=1e-8), [5.0, 0.0, 1.0], 1.4e5, 50.0, rapt.m_pr, rapt.e,
+            2e-5, 4e-4),
+           ("twin_gc", U["TwinDipole"](), list(map(float, wl["pos"][1])), float(wl["v"][1]),
+            float(wl["pa"][1]), rapt.m_el, -rapt.e, 0.01, 1.0),
+           ("breathing_gc", U["BreathingSheet"](), [1.0 * Re, 2.0 * Re, 0.9 * Re], 4.0e5, 40.0,
+            rapt.m_pr, rapt.e, 0.5, 5.0)]
+    for name, f, pos, v, pa, m, q, gdt, delta in gcs:
+        reset_params(GCtimestep=gdt)
+        g = rapt.GuidingCenter(pos, v, pa, None, 0, m, q, f)
+        g.advance(delta)
+        out[name + "/in"] = np.r_[pos, v, pa, m, q, g.mu, gdt, delta]
+        out[name + "/traj"] = g.trajectory.copy()
+        out[name + "/tcur"] = np.array([g.tcur])
+    reset_params()
+    np.savez_compressed(os.path.join(HERE, "userfields.npz"), **out)
+
+
 ALL = (make_fields, make_particles, make_gcs, make_utils, make_rkf, make_bounce,
-       make_switching, make_accessors)
+       make_switching, make_accessors, make_userfields)
 
 if __name__ == "__main__":
     # with arguments, only the named fixtures (e.g. make_switching)

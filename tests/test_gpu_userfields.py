@@ -1,0 +1,191 @@
+"""User-written fields on the device: a ``_Field`` subclass overriding
+``B``/``E`` in Python (rapt/fields.py:5-74), traced into a register program
+that the kernels interpret in FP64.  Checked against the reference's values
+and trajectories for the same class bodies (tests/golden/userfields.npz) and
+against the oracle, which interprets the same program with the C library's
+math and matches the reference bit for bit on these cases.
+
+Tolerances: fields within 4e-15 relative of the reference (the device's pow,
+exp, tanh, sin, cos and cosh differ from the C library's by an ulp or two);
+finite-difference derivatives within 1e-8 of the oracle (they amplify those
+ulps by scale/step); trajectories at the output times within 1e-7 relative in
+position, the bar of every adaptive comparison.
+"""
+import numpy as np
+import pytest
+
+import rapt_amd as rapt
+from rapt_amd import _native as N
+from rapt_amd import batch
+from rapt_amd import constants as K
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = dict(rapt.params)
+
+
+def P(**kw):
+    d = dict(PARAMS)
+    d.update(kw)
+    return d
+
+
+def pos_err(a, b):
+    d = np.linalg.norm(a[:, 1:4] - b[:, 1:4], axis=1)
+    return np.max(d / np.linalg.norm(b[:, 1:4], axis=1))
+
+
+FIELDS = {"ChargedDipole": lambda U: U["ChargedDipole"](),
+          "TwinDipole": lambda U: U["TwinDipole"](),
+          "BreathingSheet": lambda U: U["BreathingSheet"]()}
+
+
+@pytest.mark.parametrize("name", sorted(FIELDS))
+def test_device_field_values(oracle, golden, user_fields, name):
+    g = golden("userfields")
+    f = FIELDS[name](user_fields)
+    d = f.descriptor()
+    tp = g[name + "/tpos"]
+    ctx = N.context()
+    for w in ("B", "E"):
+        ref = g[name + "/" + w]
+        got = ctx.field_eval(d, w, tp)
+        np.testing.assert_allclose(got, ref, rtol=4e-15, atol=4e-15 * np.max(np.abs(ref)),
+                                   err_msg=w)
+    fo = oracle.field_from_desc(d)
+    for w in ("magB", "gradB", "curvature", "lengthscale", "dBdt", "jacobianB", "curlb"):
+        if w == "dBdt" and f.static:
+            continue
+        ref = oracle.field_eval(fo, w, tp)
+        got = ctx.field_eval(d, w, tp)
+        np.testing.assert_allclose(got, ref, rtol=1e-8, atol=1e-8 * np.max(np.abs(ref)),
+                                   err_msg=w)
+
+
+def test_particle_object_in_charged_dipole(golden, user_fields):
+    """The reference's new-field example through the object API: a 1 eV
+    proton at 5 m in a charged dipole, setke(1), advance(4e-4)."""
+    g = golden("userfields")
+    old = dict(rapt.params)
+    try:
+        rapt.params.clear()
+        rapt.params.update(PARAMS)
+        p = rapt.Particle([5, 0, 0], [0, 1, 0], 0, rapt.m_pr, rapt.e,
+                          user_fields["ChargedDipole"](Q=1e-6))
+        p.setke(1)
+        p.advance(4e-4)
+    finally:
+        rapt.params.clear()
+        rapt.params.update(old)
+    ref = g["cd_particle/traj"]
+    assert p.trajectory.shape == ref.shape
+    assert pos_err(p.trajectory, ref) < 1e-7
+    assert p.tcur == g["cd_particle/tcur"][0]
+
+
+@pytest.mark.parametrize("name,cls", [("twin_particle", "TwinDipole"),
+                                      ("breathing_particle", "BreathingSheet")])
+def test_particle_user_fields_vs_reference(golden, user_fields, name, cls):
+    g = golden("userfields")
+    ref = g[name + "/traj"]
+    m, q, delta = g[name + "/in"][6:9]
+    res = batch.advance_particles(user_fields[cls](), ref[:1], m, q, delta, params=P())
+    assert res.status[0] == N.ST_DONE
+    got = np.vstack([ref[:1], res.rows[0]])
+    assert got.shape == ref.shape
+    assert pos_err(got, ref) < 1e-7
+
+
+@pytest.mark.parametrize("name,make", [
+    ("cd_gc", lambda U: U["ChargedDipole"](Q=1e-8)),
+    ("twin_gc", lambda U: U["TwinDipole"]()),
+    ("breathing_gc", lambda U: U["BreathingSheet"]()),
+])
+def test_gc_user_fields_vs_reference(golden, user_fields, name, make):
+    """GuidingCenter.advance in user fields: one with E, one whose B varies in
+    time (dB/dt by central differences in t)."""
+    g = golden("userfields")
+    ref = g[name + "/traj"]
+    v, pa, m, q, mu, gdt, delta = g[name + "/in"][3:]
+    res = batch.advance_gcs(make(user_fields), ref[:1], m, q, [mu], [v], [gdt], delta,
+                            params=P(GCtimestep=gdt))
+    assert res.status[0] == N.ST_DONE
+    got = np.vstack([ref[:1], res.rows[0]])
+    assert got.shape == ref.shape
+    assert pos_err(got, ref) < 1e-7
+
+
+def test_gc_object_user_field(golden, user_fields):
+    """GuidingCenter through the object API with GCtimestep set (the default
+    output step needs a field-line trace, which user fields do not take)."""
+    g = golden("userfields")
+    ref = g["breathing_gc/traj"]
+    pos, (v, pa, m, q, mu, gdt, delta) = g["breathing_gc/in"][:3], g["breathing_gc/in"][3:]
+    old = dict(rapt.params)
+    try:
+        rapt.params.clear()
+        rapt.params.update(P(GCtimestep=gdt))
+        gc = rapt.GuidingCenter(list(pos), v, pa, None, 0, m, q,
+                                user_fields["BreathingSheet"]())
+        gc.advance(delta)
+    finally:
+        rapt.params.clear()
+        rapt.params.update(old)
+    assert gc.trajectory.shape == ref.shape
+    assert pos_err(gc.trajectory, ref) < 1e-7
+
+
+def test_charged_dipole_batch_vs_oracle(oracle, user_fields):
+    """64 protons (1-100 eV, 3-8 m) in the charged dipole, one launch, each
+    against the oracle in the same traced field."""
+    rng = np.random.default_rng(17)
+    n = 64
+    r = rng.uniform(3.0, 8.0, n)
+    th = np.arccos(rng.uniform(-0.6, 0.6, n))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    ke = 10 ** rng.uniform(0, 2, n)
+    v = np.sqrt(2 * ke * K.e / K.m_pr)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    last = np.zeros((n, 7))
+    last[:, 1:4] = np.c_[r * np.sin(th) * np.cos(ph), r * np.sin(th) * np.sin(ph),
+                         r * np.cos(th)]
+    last[:, 4:7] = K.m_pr * v[:, None] * d
+    f = user_fields["ChargedDipole"](Q=1e-7)
+    res = batch.advance_particles(f, last, K.m_pr, K.e, 1e-4, params=P())
+    fo = oracle.field_from_desc(f.descriptor())
+    worst = 0.0
+    for i in range(n):
+        rows_o, info = oracle.particle_advance(fo, oracle.particle_params(), last[i], K.m_pr,
+                                               K.e, 1e-4)
+        assert res.status[i] == N.ST_DONE and info["status"] == 1, i
+        assert res.rows[i].shape == rows_o.shape, i
+        worst = max(worst, pos_err(res.rows[i], rows_o))
+    assert worst < 1e-7, worst
+
+
+def test_program_switch_between_launches(user_fields):
+    """Two user fields in turn on one context: each launch runs the program
+    of the field it was given (a stale program would give the other field's
+    numbers)."""
+    ctx = N.context()
+    tp = np.array([[0.0, 3.0, 1.0, 0.5], [1.0, 4.0, -2.0, 1.0]])
+    a, b = user_fields["ChargedDipole"](Q=1e-6), user_fields["ChargedDipole"](Q=3e-6)
+    ea = ctx.field_eval(a.descriptor(), "E", tp)
+    eb = ctx.field_eval(b.descriptor(), "E", tp)
+    ea2 = ctx.field_eval(a.descriptor(), "E", tp)
+    np.testing.assert_array_equal(ea, ea2)
+    np.testing.assert_allclose(eb, 3 * ea, rtol=1e-15)
+
+
+def test_user_field_rejections(user_fields):
+    """What the device does not take for a user field fails loudly: FP32
+    field precision and the field-line integrals."""
+    f = user_fields["TwinDipole"]()
+    last = np.zeros((1, 7))
+    last[0, 1:4] = [5 * K.Re, 0, 0]
+    last[0, 5] = K.m_pr * 1e7
+    with pytest.raises(RuntimeError, match="FP64 only"):
+        batch.advance_particles(f, last, K.m_pr, K.e, 1.0, params=P(), precision="fp32_field")
+    with pytest.raises(RuntimeError, match="dipole-type"):
+        N.context().halfbounce(f.descriptor(), last[:, :4], np.array([1e-6]), 1e-3)

@@ -39,14 +39,19 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(N.Counters) == 7 * 8
 
 
-def test_descriptor_rejects_python_only_fields():
+def test_descriptor_traces_python_overrides():
     class MyField(F._Field):
         def B(self, tpos):
             return np.array([0.0, 0.0, 1e-7])
+    assert MyField().descriptor()["model"] == F.FIELD_EXPR
+    d = type("E2", (F.UniformBz,), {"E": lambda self, t: np.ones(3)})().descriptor()
+    assert d["model"] == F.FIELD_EXPR and d["expr"].ne > 0
+
+    class Branchy(F._Field):
+        def B(self, tpos):
+            return np.array([0.0, 0.0, 1.0 if tpos[3] > 0 else 2.0])
     with pytest.raises(NotImplementedError):
-        MyField().descriptor()
-    with pytest.raises(NotImplementedError):
-        type("E2", (F.UniformBz,), {"E": lambda self, t: np.ones(3)})().descriptor()
+        Branchy().descriptor()
     d = F.DoubleDipole().descriptor()
     assert d["model"] == 2 and d["gstep"] == K.Re / 1000
 
