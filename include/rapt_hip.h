@@ -255,6 +255,20 @@ int rapt_bounce_center_advance(rapt_ctx *ctx, int64_t n, double *tpos, const dou
                                double eyegradientstep, double fieldlineresolution,
                                int32_t nsteps);
 
+/* The equatorial branch of BounceCenter.advance (isequatorial = True,
+ * rapt/BounceCenter.py:237-238): dopri5 with scipy's call semantics on the
+ * gradient plus inertial drift, _v_grad + _v_inert (:154-204), one
+ * integrate() per output row, rows as rapt_bounce_center_advance.
+ * k [n][4] = (mu, mass, charge, gamma) per centre.  The reference calls the
+ * two drifts with the 3-vector (x, y, z), which every field's `t, x, y, z =
+ * tpos` rejects, so its branch raises; here they take (t, x, y, z).  Any
+ * static field (models 0-4 and 6-10); status 1 ok, -2 nsteps, -3 step too
+ * small. */
+int rapt_bounce_center_equatorial(rapt_ctx *ctx, int64_t n, double *tpos, const double *k,
+                                  const double *dt, const double *tcur, const int64_t *nrow,
+                                  double *rows, int32_t *status, int64_t *naccpt, double rtol,
+                                  double atol, int32_t nsteps);
+
 /* Guiding centre and first invariant of every row of Particle trajectories:
  * Particle.guidingcenter() and Particle.mu() (rapt/Particle.py:463-482), i.e.
  * utils.guidingcenter (rapt/utils.py:251-326) and utils.magnetic_moment

@@ -76,6 +76,14 @@ def lib():
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                        C.POINTER(Counters)]
         L.orc_adaptive_run.restype = C.c_int64
+        L.orc_bc_equatorial.argtypes = [C.POINTER(Field), C.POINTER(Params), dp, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_int64, dp, C.POINTER(C.c_int32),
+                                        C.POINTER(Counters)]
+        L.orc_bc_equatorial.restype = C.c_int64
+        L.orc_bc_equatorial_drifts.argtypes = [C.POINTER(Field), dp, C.c_double, C.c_double,
+                                               C.c_double, C.c_double, dp]
+        L.orc_bc_equatorial_drifts.restype = None
         L.orc_adaptive_continue.argtypes = [C.POINTER(Field), C.POINTER(Params), C.c_int, dp,
                                             C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_double, dp, C.c_int64, C.POINTER(C.c_int32),
@@ -244,6 +252,26 @@ def gc_advance(field, params, eom, last, mass, charge, mu, v, dt, delta, cap=400
                              C.byref(sw), C.byref(cnt))
     return rows[:n].copy(), dict(tcur=tcur.value, status=st.value, switched=sw.value,
                                  naccpt=cnt.naccpt, nrejct=cnt.nrejct, nfcn=cnt.nfcn)
+
+
+def bc_equatorial_drifts(field, tpos, mu, mass, charge, gamma):
+    """(_v_grad, _v_inert) of BounceCenter at one (t, x, y, z)."""
+    tp = np.ascontiguousarray(tpos, dtype=np.float64)
+    out = np.zeros(6)
+    lib().orc_bc_equatorial_drifts(C.byref(field), _dp(tp), mu, mass, charge, gamma, _dp(out))
+    return out[:3].copy(), out[3:].copy()
+
+
+def bc_equatorial(field, params, tpos, mu, mass, charge, gamma, dt, tcur, nrow):
+    """BounceCenter.advance with isequatorial: rows (nrow, 4), end state, info."""
+    tp = np.ascontiguousarray(tpos, dtype=np.float64).copy()
+    rows = np.zeros((max(int(nrow), 1), 4))
+    st = C.c_int32()
+    cnt = Counters()
+    n = lib().orc_bc_equatorial(C.byref(field), C.byref(params), _dp(tp), mu, mass, charge,
+                                gamma, dt, tcur, int(nrow), _dp(rows), C.byref(st),
+                                C.byref(cnt))
+    return rows[:n].copy(), tp, dict(status=st.value, naccpt=cnt.naccpt, nfcn=cnt.nfcn)
 
 
 def adaptive_run(field, params, pos, vel, t0, mass, charge, delta, cap=400000):

@@ -1394,3 +1394,85 @@ int orc_batch_gc(const orc_field *f, const orc_params *p, int eom, int64_t n, co
     if (cnt) { cnt->naccpt = acc; cnt->nrejct = rej; cnt->nstep = nst; cnt->nfcn = nfc; }
     return 0;
 }
+
+/* ======================================================================
+ * BounceCenter, equatorial branch: rapt/BounceCenter.py:154-204, 237-238
+ * ==================================================================== */
+typedef struct {
+    const orc_field *f;
+    double mu, mass, charge, gamma;
+} eqctx;
+
+/* _v_grad (:154-172): mu (B x gradB) / (q gamma |B|^2) */
+static void eq_vgrad(const eqctx *c, const double *tp, double *out) {
+    double B[3], g[3], x[3];
+    field_B(c->f, tp, B);
+    const double bsq = dot3(B, B);
+    field_gradB(c->f, tp, g);
+    cross3(B, g, x);
+    const double den = c->charge * c->gamma * bsq;
+    for (int k = 0; k < 3; ++k) out[k] = c->mu * x[k] / den;
+}
+
+/* _v_inert (:174-204): the 4-vector tpos shifted by d (t, vg/|vg|) */
+static void eq_vinert(const eqctx *c, const double *tp, double *out) {
+    double vg[3], v1[3], v2[3], B[3], dv[3], x[3], dir[4], tm[4], tq[4];
+    eq_vgrad(c, tp, vg);
+    const double vm = sqrt(dot3(vg, vg));
+    dir[0] = tp[0];
+    for (int k = 0; k < 3; ++k) dir[k + 1] = vg[k] / vm;
+    const double d = c->f->gstep;
+    for (int k = 0; k < 4; ++k) {
+        tm[k] = tp[k] - d * dir[k];
+        tq[k] = tp[k] + d * dir[k];
+    }
+    eq_vgrad(c, tm, v1);
+    eq_vgrad(c, tq, v2);
+    field_B(c->f, tp, B);
+    const double bsq = dot3(B, B);
+    for (int k = 0; k < 3; ++k) dv[k] = v2[k] - v1[k];
+    cross3(B, dv, x);
+    const double s = c->mass * vm / (c->charge * bsq * 2 * d);
+    for (int k = 0; k < 3; ++k) out[k] = s * x[k];
+}
+
+/* deriv's equatorial branch with tpos = (t, Y): _v_grad + _v_inert */
+static void eq_rhs(void *vc, double t, const double *y, double *out) {
+    const eqctx *c = (const eqctx *)vc;
+    const double tp[4] = {t, y[0], y[1], y[2]};
+    double vg[3], vi[3];
+    eq_vgrad(c, tp, vg);
+    eq_vinert(c, tp, vi);
+    for (int k = 0; k < 3; ++k) out[k] = vg[k] + vi[k];
+}
+
+/* _v_grad and _v_inert at one 4-vector tpos: out[0..2], out[3..5] */
+void orc_bc_equatorial_drifts(const orc_field *f, const double tpos[4], double mu, double mass,
+                              double charge, double gamma, double *out) {
+    eqctx c = {f, mu, mass, charge, gamma};
+    eq_vgrad(&c, tpos, out);
+    eq_vinert(&c, tpos, out + 3);
+}
+
+int64_t orc_bc_equatorial(const orc_field *f, const orc_params *p, double tpos[4], double mu,
+                          double mass, double charge, double gamma, double dt, double tcur,
+                          int64_t nrow, double *rows, int32_t *status, orc_counters *cnt) {
+    eqctx c = {f, mu, mass, charge, gamma};
+    double x = tpos[0], y[3] = {tpos[1], tpos[2], tpos[3]};
+    orc_ode st;
+    memset(&st, 0, sizeof st);
+    st.h = p->first_step;
+    int rc = 1;
+    int64_t r = 0;
+    for (; r < nrow; ++r) {
+        rc = orc_dopri5_call(eq_rhs, &c, 3, &x, y, x + dt, p, &st);
+        if (rc < 0) break;
+        double *row = rows + 4 * r;
+        row[0] = tcur + (double)r * dt;
+        row[1] = y[0]; row[2] = y[1]; row[3] = y[2];
+    }
+    tpos[0] = x; tpos[1] = y[0]; tpos[2] = y[1]; tpos[3] = y[2];
+    if (status) *status = rc;
+    if (cnt) *cnt = st.cnt;
+    return r;
+}

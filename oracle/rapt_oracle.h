@@ -199,6 +199,16 @@ int orc_batch_gc(const orc_field *f, const orc_params *p, int eom, int64_t n, co
                  const int64_t *row_cap, int64_t *nrows, int32_t *status, orc_counters *cnt,
                  int nthreads);
 
+/* BounceCenter.advance with isequatorial (rapt/BounceCenter.py:154-204,
+ * 237-238): dopri5 (p) on _v_grad + _v_inert at (t, x, y, z), nrow
+ * integrate() calls of dt; rows [nrow][4] with the reference's row times
+ * tcur + r dt.  tpos in/out; returns rows written. */
+void orc_bc_equatorial_drifts(const orc_field *f, const double tpos[4], double mu, double mass,
+                              double charge, double gamma, double *out);
+int64_t orc_bc_equatorial(const orc_field *f, const orc_params *p, double tpos[4], double mu,
+                          double mass, double charge, double gamma, double dt, double tcur,
+                          int64_t nrow, double *rows, int32_t *status, orc_counters *cnt);
+
 #ifdef __cplusplus
 }
 #endif

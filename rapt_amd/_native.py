@@ -96,6 +96,8 @@ def lib():
     L.rapt_bounce_center_advance.argtypes = [vp, C.c_int64, dp, dp, dp, dp, dp, i64p, dp, i32p,
                                              i64p, C.c_double, C.c_double, C.c_double,
                                              C.c_double, C.c_int32]
+    L.rapt_bounce_center_equatorial.argtypes = [vp, C.c_int64, dp, dp, dp, dp, i64p, dp, i32p,
+                                                i64p, C.c_double, C.c_double, C.c_int32]
     fp = C.POINTER(C.c_float)
     L.rapt_mlp_create.argtypes = [vp, C.c_int32, fp, C.c_double, C.c_double, C.POINTER(vp)]
     L.rapt_mlp_destroy.argtypes = [vp]
@@ -128,6 +130,7 @@ def lib():
                                             vp, vp, vp, vp, vp, vp, C.POINTER(Counters)]
     for nm in ("rapt_ctx_create", "rapt_ctx_destroy", "rapt_field_set", "rapt_field_eval",
                "rapt_halfbounce", "rapt_bounce_labels", "rapt_bounce_center_advance",
+               "rapt_bounce_center_equatorial",
                "rapt_batch_epochs_begin", "rapt_batch_epoch_next", "rapt_batch_epochs_end",
                "rapt_mlp_create", "rapt_mlp_destroy",
                "rapt_mlp_set_norm", "rapt_mlp_set_split", "rapt_mlp_targets", "rapt_mlp_train", "rapt_mlp_get_params",
@@ -297,6 +300,26 @@ class Context:
             self.h, n, _dp(tp), _dp(Bm), _dp(coef), _dp(dt), _dp(tcur), _ip64(nrow), _dp(rows),
             _ip32(st), _ip64(acc), float(rtol), float(atol), float(eyegradientstep),
             float(fieldlineresolution), int(nsteps)), "rapt_bounce_center_advance")
+        off = np.r_[0, np.cumsum(nrow)]
+        return [rows[off[i]:off[i + 1]] for i in range(n)], tp, st, acc
+
+
+    def bounce_center_equatorial(self, desc, tpos, k, dt, tcur, nrow, rtol, atol, nsteps=500):
+        """Equatorial bounce centres (gradient + inertial drift); k: (n, 4) rows of
+        (mu, mass, charge, gamma).  Returns (rows list, final tpos, status, naccpt)."""
+        self.set_field(desc)
+        tp = np.ascontiguousarray(np.atleast_2d(tpos), dtype=np.float64).copy()
+        n = tp.shape[0]
+        k = np.ascontiguousarray(np.broadcast_to(k, (n, 4)), dtype=np.float64)
+        f = lambda a: np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.float64)
+        dt, tcur = f(dt), f(tcur)
+        nrow = np.ascontiguousarray(np.broadcast_to(nrow, (n,)), dtype=np.int64)
+        rows = np.zeros((max(int(nrow.sum()), 1), 4))
+        st = np.zeros(n, np.int32)
+        acc = np.zeros(n, np.int64)
+        self.check(lib().rapt_bounce_center_equatorial(
+            self.h, n, _dp(tp), _dp(k), _dp(dt), _dp(tcur), _ip64(nrow), _dp(rows), _ip32(st),
+            _ip64(acc), float(rtol), float(atol), int(nsteps)), "rapt_bounce_center_equatorial")
         off = np.r_[0, np.cumsum(nrow)]
         return [rows[off[i]:off[i + 1]] for i in range(n)], tp, st, acc
 

@@ -543,17 +543,57 @@ struct BCArgs {
   CtlParams c5;
 };
 
+// The equatorial branch of deriv (rapt/BounceCenter.py:237-238): gradient
+// plus inertial drift, _v_grad + _v_inert (:154-204), for bounce centres
+// with no parallel motion.  _v_grad = mu (B x grad|B|) / (q gamma |B|^2);
+// _v_inert = m |v_g| / (q |B|^2 2d) B x (v_g(+) - v_g(-)), the gradient drift
+// differenced at +-d along its own direction, where the reference shifts the
+// whole 4-vector (t, x, y, z) by d (t, v_g/|v_g|) -- the time shift included.
+// The reference calls both with Y[:4], the 3-vector (x, y, z), which the
+// fields' `t, x, y, z = tpos` rejects; here they take (t, x, y, z).
+template <int M>
+struct EqDriftRHS {
+  const FieldC& F;
+  double mu, mass, charge, gamma;
+  int32_t st;
+  int64_t nf;
+  __device__ __forceinline__ V3 vgrad(double t, double x, double y, double z) const {
+    const V3 B = fieldB<M>(F, t, x, y, z);
+    const double bsq = dot3(B, B);
+    const double d = F.gstep;
+    // _Field.gradB (rapt/fields.py:111-131): central differences of |B|
+    const double gx = (magB<M>(F, t, x + d, y, z) - magB<M>(F, t, x - d, y, z)) / (2.0 * d);
+    const double gy = (magB<M>(F, t, x, y + d, z) - magB<M>(F, t, x, y - d, z)) / (2.0 * d);
+    const double gz = (magB<M>(F, t, x, y, z + d) - magB<M>(F, t, x, y, z - d)) / (2.0 * d);
+    const V3 c = cross3(B, V3{gx, gy, gz});
+    const double den = charge * gamma * bsq;
+    return V3{mu * c.x / den, mu * c.y / den, mu * c.z / den};
+  }
+  __device__ __forceinline__ void operator()(double t, const double* Y, double* out) {
+    ++nf;
+    const V3 vg = vgrad(t, Y[0], Y[1], Y[2]);
+    const double vm = __builtin_sqrt(dot3(vg, vg));
+    const double d = F.gstep;
+    const double ux = vg.x / vm, uy = vg.y / vm, uz = vg.z / vm;
+    const V3 v1 = vgrad(t - d * t, Y[0] - d * ux, Y[1] - d * uy, Y[2] - d * uz);
+    const V3 v2 = vgrad(t + d * t, Y[0] + d * ux, Y[1] + d * uy, Y[2] + d * uz);
+    const V3 B = fieldB<M>(F, t, Y[0], Y[1], Y[2]);
+    const double s = mass * vm / (charge * dot3(B, B) * 2.0 * d);
+    const V3 c = cross3(B, V3{v2.x - v1.x, v2.y - v1.y, v2.z - v1.z});
+    out[0] = vg.x + s * c.x;
+    out[1] = vg.y + s * c.y;
+    out[2] = vg.z + s * c.z;
+  }
+};
+
 // One lane per bounce centre: r = ode(deriv).set_integrator("dopri5", ...),
 // then for each t in arange(tcur, tcur + delta, dt): integrate(r.t + dt) and
 // append [t, r.y] (the row's time is the arange value, one step behind the
 // state, as the reference writes it).  scipy's call semantics as in k_trace.
-template <int M>
-__global__ __launch_bounds__(256) void k_bounce_center(BounceArgs A, BCArgs C) {
-  const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (lane >= A.lanes) return;
+template <class RHS>
+__device__ __forceinline__ void bc_run(RHS& f, const BCArgs& C, int64_t i) {
   const CtlParams& P = C.c5;
-  for (int64_t i = lane; i < C.n; i += A.lanes) {
-    DriftRHS<M> f{A, lane, C.Bm[i], C.coef[i], C.dstep, BST_OK, 0};
+  {
     double x = C.tpos[4 * i];
     double y[3] = {C.tpos[4 * i + 1], C.tpos[4 * i + 2], C.tpos[4 * i + 3]};
     double k1[3], y1[3], k7[3];
@@ -620,6 +660,29 @@ __global__ __launch_bounds__(256) void k_bounce_center(BounceArgs A, BCArgs C) {
     C.status[i] = code;
     C.naccpt[i] = acc;
   }
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_bounce_center(BounceArgs A, BCArgs C) {
+  const int64_t lane = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= A.lanes) return;
+  for (int64_t i = lane; i < C.n; i += A.lanes) {
+    DriftRHS<M> f{A, lane, C.Bm[i], C.coef[i], C.dstep, BST_OK, 0};
+    bc_run(f, C, i);
+  }
+}
+
+// Per-centre constants of the equatorial drift: mu, mass, charge, gamma
+struct EqArgs {
+  const double* k;  // [n][4]
+};
+
+template <int M>
+__global__ __launch_bounds__(64) void k_bounce_center_eq(FieldC F, BCArgs C, EqArgs E) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= C.n) return;
+  EqDriftRHS<M> f{F, E.k[4 * i], E.k[4 * i + 1], E.k[4 * i + 2], E.k[4 * i + 3], BST_OK, 0};
+  bc_run(f, C, i);
 }
 
 }  // namespace rapt
@@ -889,5 +952,125 @@ extern "C" int rapt_bounce_center_advance(rapt_ctx* c, int64_t n, double* tpos, 
   (void)hipFree(d_acc);
   (void)hipFree(d_st);
   (void)hipFree(scratch);
+  return rc;
+}
+
+template <int M>
+static void launch_bc_eq(rapt_ctx* c, const BCArgs& C, const EqArgs& E) {
+  const int B = 64;
+  hipLaunchKernelGGL(k_bounce_center_eq<M>, dim3((unsigned)((C.n + B - 1) / B)), dim3(B), 0,
+                     c->stream, c->field, C, E);
+}
+
+extern "C" int rapt_bounce_center_equatorial(rapt_ctx* c, int64_t n, double* tpos,
+                                             const double* k, const double* dt,
+                                             const double* tcur, const int64_t* nrow,
+                                             double* rows, int32_t* status, int64_t* naccpt,
+                                             double rtol, double atol, int32_t nsteps) {
+  if (!c || n < 0 || (n && (!tpos || !k || !dt || !tcur || !nrow || !rows || !status ||
+                            !naccpt)))
+    return RAPT_E_ARG;
+  if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
+  if (!c->field.is_static)
+    return set_err(c, RAPT_E_ARG, "BounceCenter does not work with nonstatic fields");
+  if (!(rtol > 0.0) || !(atol > 0.0)) return set_err(c, RAPT_E_ARG, "bad BounceCenter tolerances");
+  if (n == 0) return RAPT_OK;
+  if (!all_finite(tpos, 4 * n) || !all_finite(k, 4 * n) || !all_finite(tcur, n))
+    return set_err(c, RAPT_E_ARG, "non-finite BounceCenter state or constants");
+  int64_t total = 0;
+  std::vector<int64_t> off((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (nrow[i] < 0 || !(dt[i] > 0.0)) return set_err(c, RAPT_E_ARG, "bad row count or step");
+    off[(size_t)i] = total;
+    total += nrow[i];
+  }
+  HIP_OK(c, hipSetDevice(c->device));
+  double *d_tpos = nullptr, *d_in = nullptr, *d_rows = nullptr;
+  int64_t *d_nrow = nullptr, *d_off = nullptr, *d_acc = nullptr;
+  int32_t* d_st = nullptr;
+  int rc = RAPT_OK;
+  if (hipMalloc(&d_tpos, sizeof(double) * 4 * n) != hipSuccess ||
+      hipMalloc(&d_in, sizeof(double) * 6 * n) != hipSuccess ||
+      hipMalloc(&d_rows, sizeof(double) * 4 * (total > 0 ? total : 1)) != hipSuccess ||
+      hipMalloc(&d_nrow, sizeof(int64_t) * n) != hipSuccess ||
+      hipMalloc(&d_off, sizeof(int64_t) * n) != hipSuccess ||
+      hipMalloc(&d_acc, sizeof(int64_t) * n) != hipSuccess ||
+      hipMalloc(&d_st, sizeof(int32_t) * n) != hipSuccess) {
+    rc = set_err(c, RAPT_E_OOM, "BounceCenter allocation failed");
+  } else {
+    BCArgs C;
+    memset(&C, 0, sizeof C);
+    C.n = n;
+    C.tpos = d_tpos;
+    C.dt = d_in + 4 * n;
+    C.tcur = d_in + 5 * n;
+    C.nrow = d_nrow;
+    C.row_off = d_off;
+    C.rows = d_rows;
+    C.status = d_st;
+    C.naccpt = d_acc;
+    C.rtol = rtol;
+    C.atol = atol;
+    C.nmax = nsteps > 0 ? nsteps : 500;
+    // scipy's dopri5 defaults: safety 0.9, fac1 0.2, fac2 10, beta 0 -> 0.04
+    C.c5.safe = 0.9;
+    C.c5.facc1 = 1.0 / 0.2;
+    C.c5.facc2 = 1.0 / 10.0;
+    C.c5.beta = 0.04;
+    C.c5.expo1 = 0.2 - 0.04 * 0.75;
+    const EqArgs E{d_in};
+    const bool ok =
+        hipMemcpyAsync(d_tpos, tpos, sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->stream) ==
+            hipSuccess &&
+        hipMemcpyAsync(d_in, k, sizeof(double) * 4 * n, hipMemcpyHostToDevice, c->stream) ==
+            hipSuccess &&
+        hipMemcpyAsync(d_in + 4 * n, dt, sizeof(double) * n, hipMemcpyHostToDevice, c->stream) ==
+            hipSuccess &&
+        hipMemcpyAsync(d_in + 5 * n, tcur, sizeof(double) * n, hipMemcpyHostToDevice,
+                       c->stream) == hipSuccess &&
+        hipMemcpyAsync(d_nrow, nrow, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream) ==
+            hipSuccess &&
+        hipMemcpyAsync(d_off, off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice,
+                       c->stream) == hipSuccess;
+    if (!ok) {
+      rc = set_err(c, RAPT_E_HIP, "BounceCenter input copy failed");
+    } else {
+      switch (c->field.model) {
+        case 0: launch_bc_eq<0>(c, C, E); break;
+        case 1: launch_bc_eq<1>(c, C, E); break;
+        case 2: launch_bc_eq<2>(c, C, E); break;
+        case 3: launch_bc_eq<3>(c, C, E); break;
+        case 4: launch_bc_eq<4>(c, C, E); break;
+        case 6: launch_bc_eq<6>(c, C, E); break;
+        case 7: launch_bc_eq<7>(c, C, E); break;
+        case 8: launch_bc_eq<8>(c, C, E); break;
+        case 9: launch_bc_eq<9>(c, C, E); break;
+        case 10: launch_bc_eq<10>(c, C, E); break;
+        default:
+          rc = set_err(c, RAPT_E_ARG, "BounceCenter needs a static field (model %d)",
+                       c->field.model);
+      }
+      if (rc == RAPT_OK) {
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(tpos, d_tpos, sizeof(double) * 4 * n, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            (total > 0 && hipMemcpyAsync(rows, d_rows, sizeof(double) * 4 * total,
+                                         hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
+            hipMemcpyAsync(status, d_st, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipMemcpyAsync(naccpt, d_acc, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+          rc = set_err(c, RAPT_E_HIP, "BounceCenter kernel failed");
+      }
+    }
+  }
+  (void)hipFree(d_tpos);
+  (void)hipFree(d_in);
+  (void)hipFree(d_rows);
+  (void)hipFree(d_nrow);
+  (void)hipFree(d_off);
+  (void)hipFree(d_acc);
+  (void)hipFree(d_st);
   return rc;
 }

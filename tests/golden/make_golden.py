@@ -473,8 +473,54 @@ def make_userfields():
     np.savez_compressed(os.path.join(HERE, "userfields.npz"), **out)
 
 
+def make_equatorial():
+    """BounceCenter's equatorial drifts, _v_grad and _v_inert
+    (rapt/BounceCenter.py:154-204), at 4-vector positions, and advance()
+    with deriv's equatorial branch (:237-238) given the tpos that deriv
+    builds: the reference passes Y[:4], the 3-vector (x, y, z), which every
+    field's ``t, x, y, z = tpos`` rejects, so the loop of advance()
+    (:206-251) is run here as written with that one argument changed."""
+    from scipy.integrate import ode
+    out = {}
+    cases = [("dip_p", rf.EarthDipole(), [4 * Re * np.cos(0.3), 4 * Re * np.sin(0.3), 0.0],
+              ru.speedfromKE(1e6, rapt.m_pr), 90.0, rapt.m_pr, rapt.e, 6.0),
+             ("dd_e", rf.DoubleDipole(), [6 * Re, 1 * Re, 0.0], ru.speedfromKE(1e5, rapt.m_el),
+              60.0, rapt.m_el, -rapt.e, 4.0)]
+    for name, f, pos, v, pa, m, q, nsteps in cases:
+        reset_params()
+        bc = rapt.BounceCenter(pos, float(v), 0, pa, m, q, f)
+        out[name + "/in"] = np.r_[pos, v, pa, m, q, bc.mu]
+        rng = np.random.default_rng(3)
+        pts = np.array([[rng.uniform(0, 5), pos[0] + rng.uniform(-0.3, 0.3) * Re,
+                         pos[1] + rng.uniform(-0.3, 0.3) * Re, rng.uniform(-0.2, 0.2) * Re]
+                        for _ in range(8)])
+        out[name + "/tpos"] = pts
+        out[name + "/v_grad"] = np.array([bc._v_grad(tp) for tp in pts])
+        out[name + "/v_inert"] = np.array([bc._v_inert(tp) for tp in pts])
+        gamma = 1.0 / np.sqrt(1 - (v / rapt.c) ** 2)
+        Bm = m * gamma ** 2 * v ** 2 / (2 * bc.mu)
+        bp = rfu.bounceperiod(bc.trajectory[-1, :4], f, Bm, v)
+        dt = rapt.params["BCtimestep"] * bp
+        delta = nsteps * dt
+
+        def deriv(t, Y):
+            tpos = np.concatenate(([t], Y))
+            return bc._v_grad(tpos) + bc._v_inert(tpos)
+        rtol, atol = rapt.params["solvertolerances"]
+        r = ode(deriv).set_integrator("dopri5", atol=atol, rtol=rtol)
+        r.set_initial_value(bc.trajectory[-1, 1:], bc.trajectory[-1, 0])
+        traj = bc.trajectory
+        for t in np.arange(bc.tcur, bc.tcur + delta, dt):
+            traj = np.vstack((traj, np.hstack(([t], r.integrate(r.t + dt)))))
+        out[name + "/dt"] = np.array([dt, delta, Bm, gamma])
+        out[name + "/traj"] = traj
+        out[name + "/end"] = np.r_[r.t, r.y]
+    reset_params()
+    np.savez_compressed(os.path.join(HERE, "equatorial.npz"), **out)
+
+
 ALL = (make_fields, make_particles, make_gcs, make_utils, make_rkf, make_bounce,
-       make_switching, make_accessors, make_userfields)
+       make_switching, make_accessors, make_userfields, make_equatorial)
 
 if __name__ == "__main__":
     # with arguments, only the named fixtures (e.g. make_switching)

@@ -62,3 +62,56 @@ def test_bounce_center_rejects_time_dependent_field():
     with pytest.raises(RuntimeError):
         rapt.BounceCenter([4 * K.Re, 0, 0], 1e7, 0.0, 1.0, K.m_pr, K.e,
                           F.VarEarthDipole(amp=0.1, period=5.0))
+
+
+EQ_FIELDS = {"dip_p": F.EarthDipole, "dd_e": F.DoubleDipole}
+
+
+@pytest.mark.parametrize("name", sorted(EQ_FIELDS))
+def test_equatorial_branch_vs_reference(golden, name):
+    """isequatorial (rapt/BounceCenter.py:237-238): gradient plus inertial
+    drift through the C-ABI at the reference's output step, against the
+    reference's rows (its loop given the tpos its deriv builds).  Positions
+    within 1e-8 of |r|: the inertial term differences the gradient drift
+    over +-d and carries its rounding noise (1e-3 of a term 1e-4 of v_g)."""
+    from rapt_amd import _native as N
+    g = golden("equatorial")
+    _, _, _, v, pa, m, q, mu = g[name + "/in"]
+    dt, delta, Bm, gamma = g[name + "/dt"]
+    ref = g[name + "/traj"]
+    rtol, atol = rapt.params["solvertolerances"]
+    rows, end, st, acc = N.context().bounce_center_equatorial(
+        EQ_FIELDS[name]().descriptor(), ref[0], [mu, m, q, gamma], dt, 0.0, len(ref) - 1,
+        rtol, atol)
+    assert st[0] == 1
+    got = rows[0]
+    np.testing.assert_array_equal(got[:, 0], ref[1:, 0])
+    err = np.max(np.linalg.norm(got[:, 1:] - ref[1:, 1:], axis=1) /
+                 np.linalg.norm(ref[1:, 1:], axis=1))
+    print(name, "max position error / |r|", err)
+    assert err < 1e-8
+
+
+def test_equatorial_object_api(golden):
+    """BounceCenter with isequatorial set, advance() through the object API:
+    the output step comes from the device's bounce period (within 1e-4 of the
+    reference's), the rows follow the reference's."""
+    g = golden("equatorial")
+    pos = g["dip_p/in"][:3]
+    _, _, _, v, pa, m, q, mu = g["dip_p/in"]
+    dt, delta, Bm, gamma = g["dip_p/dt"]
+    ref = g["dip_p/traj"]
+    bc = rapt.BounceCenter(list(pos), v, 0, pa, m, q, F.EarthDipole())
+    assert bc.mu == pytest.approx(mu, rel=1e-12)
+    tp = g["dip_p/tpos"][0]
+    vg, vi = g["dip_p/v_grad"][0], g["dip_p/v_inert"][0]
+    np.testing.assert_allclose(bc._v_grad(tp), vg, rtol=0, atol=1e-8 * np.linalg.norm(vg))
+    np.testing.assert_allclose(bc._v_inert(tp), vi, rtol=0, atol=5e-3 * np.linalg.norm(vi))
+    bc.isequatorial = True
+    bc.advance(delta - 0.5 * dt)
+    tr = bc.trajectory
+    assert tr.shape == ref.shape
+    np.testing.assert_allclose(tr[:, 0], ref[:, 0], rtol=1e-4, atol=0)
+    err = np.max(np.linalg.norm(tr[:, 1:] - ref[:, 1:], axis=1) /
+                 np.linalg.norm(ref[:, 1:], axis=1))
+    assert err < 1e-6

@@ -99,10 +99,21 @@ def test_every_kernel_has_metadata(sizes):
 
 
 def test_gridded_tracers_and_bounce_kernels_have_no_argument_copy(sizes):
+    # (the user-written field's instantiations, model 10, hold the
+    # interpreter's register file instead: see the next test)
     picked = {k: v for k, v in sizes.items()
-              if k.startswith("_ZN4rapt7k_traceILi9E") or "bounce" in k}
+              if (k.startswith("_ZN4rapt7k_traceILi9E") or "bounce" in k) and "ILi10E" not in k}
     assert len(picked) >= 10
     big = {k: v for k, v in picked.items() if v > 64}
+    assert not big, big
+
+
+def test_user_field_kernels_hold_only_the_interpreter(sizes):
+    """Model 10 (a traced user field): the interpreter's 32 double registers
+    (256 bytes per lane) and its call frame, no argument copy."""
+    picked = {k: v for k, v in sizes.items() if "ILi10E" in k}
+    assert len(picked) >= 5
+    big = {k: v for k, v in picked.items() if v > 256 + 320}
     assert not big, big
 
 

@@ -37,7 +37,8 @@ def test_oracle_pins_pass_under_sanitizers():
     env["OMP_NUM_THREADS"] = "2"
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider",
                         "-m", "not gpu", "tests/test_oracle.py",
-                        "tests/test_oracle_switching.py"],
+                        "tests/test_oracle_switching.py", "tests/test_oracle_equatorial.py",
+                        "tests/test_userfields.py"],
                        cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
     tail = (r.stdout + r.stderr)[-3000:]
     assert r.returncode == 0, tail
