@@ -40,6 +40,22 @@
 
 namespace rapt {
 
+// The launch's kernel-argument block, re-addressed at a point of use.  Both
+// kernels below take KArgs by value as their first argument, so it starts at
+// the kernarg segment pointer.  Arguments needed only when a particle
+// starts, finishes or writes a row are read through this pointer, made
+// opaque by an empty asm statement, so each load is issued where it is used.
+// Read through the by-value argument, every such load is hoisted to the
+// kernel's entry and all of them stay in scalar registers for the whole
+// launch, which spills them into VGPR lanes and reloads them with v_readlane
+// inside the step loop.
+using KArgsK = const __attribute__((address_space(4))) KArgs;
+__device__ __forceinline__ KArgsK* cold() {
+  KArgsK* p = (KArgsK*)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 // quad_perm control that reads lane I of this lane's group of L lanes
 // (2 bits per destination lane, lane 0 in bits 0-1)
 template <int L, int I>
@@ -397,8 +413,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   const FieldC& F = A.F;
   const int eom = SP ? (int)EOM_TCB : A.eom;
   const bool stat = SP || F.is_static != 0;
-  const int64_t n = A.n;
-  double* S = A.state;
+  const int64_t n = cold()->n;
   const int q = threadIdx.x & (L - 1);
   const int c0 = (L == 8 ? (q & 3) : q) * NC;  // first component this lane holds
   const bool owner = L < 8 || q < 4;           // the lane stores its components
@@ -420,7 +435,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   const bool eq = A.eq != 0;
 
   auto write_row = [&](double trow) {
-    double* r = A.rows + (rowbase + nrow) * W;
+    double* r = cold()->rows + (rowbase + nrow) * W;
     if (q == 0) r[0] = trow;
     if (owner) {
 #pragma unroll
@@ -430,7 +445,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
   };
   auto begin_call = [&]() {
     xend = x + dt;
-    hmax = (A.max_step == 0.0) ? xend - x : A.max_step;
+    hmax = (cold()->max_step == 0.0) ? xend - x : cold()->max_step;
     lfo = LFO0;
     nstep = 0;
     nacc_call = 0;
@@ -438,6 +453,7 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
     reject = false;
   };
   auto finish = [&](int32_t code) {
+    double* S = cold()->state;
     if (owner) {
 #pragma unroll
       for (int i = 0; i < NC; ++i) S[(1 + c0 + i) * n + pi] = y.c[i];
@@ -451,10 +467,10 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
       S[RAPT_S_TEND * n + pi] = tend;
       S[RAPT_S_GM * n + pi] = gm;
       S[RAPT_S_TCUR * n + pi] = tcur;
-      A.flags[pi] = FLAG_STARTED | FLAG_GCMODE | (code != RAPT_ST_CAP ? FLAG_FINAL : 0);
-      A.status[pi] = code;
-      A.nrows[pi] = nrow;
-      A.naccpt[pi] = p_acc;
+      cold()->flags[pi] = FLAG_STARTED | FLAG_GCMODE | (code != RAPT_ST_CAP ? FLAG_FINAL : 0);
+      cold()->status[pi] = code;
+      cold()->nrows[pi] = nrow;
+      cold()->naccpt[pi] = p_acc;
     }
     active = false;
   };
@@ -468,28 +484,29 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
           // qown, then the other two in queue order
           const int qq = k == 0 ? qown : (k == 1 ? (qown == 0 ? 1 : 0) : (qown == 2 ? 1 : 2));
           if (got < n || drained[qq]) continue;
-          const int64_t lo = qq == 0 ? 0 : (qq == 1 ? A.split : A.split2);
-          const int64_t hi = qq == 0 ? A.split : (qq == 1 ? A.split2 : n);
-          const int64_t t = (int64_t)atomicAdd(&A.work[qq == 0 ? 0 : (qq == 1 ? 6 : 7)], 1ULL);
+          const int64_t lo = qq == 0 ? 0 : (qq == 1 ? cold()->split : cold()->split2);
+          const int64_t hi = qq == 0 ? cold()->split : (qq == 1 ? cold()->split2 : n);
+          const int64_t t = (int64_t)atomicAdd(&cold()->work[qq == 0 ? 0 : (qq == 1 ? 6 : 7)], 1ULL);
           if (t < hi - lo) {
             got = lo + t;
           } else {
             drained[qq] = true;
           }
         }
-        if (got < n && A.order) got = A.order[got];
+        if (got < n && cold()->order) got = cold()->order[got];
       }
       pi = gbll0<L>(got);
       if (pi >= n) {
         exhausted = true;
-      } else if (A.flags[pi] & FLAG_FINAL) {
+      } else if (cold()->flags[pi] & FLAG_FINAL) {
         // finished in an earlier launch: no rows this time, status kept
         if (q == 0) {
-          A.nrows[pi] = 0;
-          A.naccpt[pi] = 0;
+          cold()->nrows[pi] = 0;
+          cold()->naccpt[pi] = 0;
         }
       } else {
         active = true;
+        const double* S = cold()->state;
         x = S[0 * n + pi];
 #pragma unroll
         for (int i = 0; i < NC; ++i) y.c[i] = S[(1 + c0 + i) * n + pi];
@@ -503,15 +520,15 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
         tend = S[RAPT_S_TEND * n + pi];
         gm = S[RAPT_S_GM * n + pi];
         tcur = S[RAPT_S_TCUR * n + pi];
-        const int32_t fl = A.flags[pi];
+        const int32_t fl = cold()->flags[pi];
         nrow = 0;
-        rowcap = A.row_cap[pi];
-        rowbase = A.row_off[pi];
+        rowcap = cold()->row_cap[pi];
+        rowbase = cold()->row_off[pi];
         p_acc = 0;
         k1ok = false;
         if (!(fl & FLAG_STARTED)) {
-          tend = x + A.delta;
-          h = A.first_step;
+          tend = x + cold()->delta;
+          h = cold()->first_step;
           tcur = x;
         }
         if (!(x < tend)) {
@@ -669,10 +686,10 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
           finish(RAPT_ST_NAN);
         } else {
           bool sw = false;
-          if (A.check) {
+          if (cold()->check) {
             double yy[4];
             gather4<L>(y, yy);
-            sw = !gc_isadiabatic<M>(F, A, x, yy, mass, charge, mu);
+            sw = !gc_isadiabatic<M>(F, cold()->epss, cold()->epst, x, yy, mass, charge, mu);
           }
           if (sw) {
             finish(RAPT_ST_SWITCH);
@@ -690,10 +707,10 @@ __device__ __forceinline__ void gcn_body(const KArgs& A, const int qown) {
     }
   }
   if (q == 0) {
-    atomicAdd(&A.work[1], (unsigned long long)c_acc);
-    atomicAdd(&A.work[2], (unsigned long long)c_rej);
-    atomicAdd(&A.work[3], (unsigned long long)c_stp);
-    atomicAdd(&A.work[4], (unsigned long long)c_fcn);
+    atomicAdd(&cold()->work[1], (unsigned long long)c_acc);
+    atomicAdd(&cold()->work[2], (unsigned long long)c_rej);
+    atomicAdd(&cold()->work[3], (unsigned long long)c_stp);
+    atomicAdd(&cold()->work[4], (unsigned long long)c_fcn);
   }
 }
 

@@ -32,8 +32,9 @@ __device__ __forceinline__ bool particle_isadiabatic(const FieldC& F, const KArg
 // GuidingCenter.isadiabatic (rapt/GuidingCenter.py:287-327); the cycper()
 // term keeps the reference's unnormalised ppar**2 (rapt/GuidingCenter.py:535)
 template <int M>
-__device__ __forceinline__ bool gc_isadiabatic(const FieldC& F, const KArgs& A, double t, const double* y,
-                               double mass, double charge, double mu) {
+__device__ __forceinline__ bool gc_isadiabatic(const FieldC& F, double epss, double epst, double t,
+                                               const double* y, double mass, double charge,
+                                               double mu) {
   const double Bmag = magB<M>(F, t, y[0], y[1], y[2]);
   const double pp = y[3];
   const double mcc = mass * C_LIGHT * C_LIGHT;
@@ -50,7 +51,7 @@ __device__ __forceinline__ bool gc_isadiabatic(const FieldC& F, const KArgs& A, 
   const double g2 = 1.0 / __builtin_sqrt(1.0 - (v / C_LIGHT) * (v / C_LIGHT));
   const double vperp = __builtin_sqrt((v - vp) * (v + vp));
   const double rc = g2 * mass * vperp / (__builtin_fabs(charge) * Bmag);
-  bool ok = rc / lengthscale<M>(F, t, y[0], y[1], y[2]) < A.epss;
+  bool ok = rc / lengthscale<M>(F, t, y[0], y[1], y[2]) < epss;
   if (!F.is_static && ok) {
     const double g3 = __builtin_sqrt(1.0 + 2.0 * mu * Bmag / mcc + pp * pp);
     double v3;
@@ -62,7 +63,7 @@ __device__ __forceinline__ bool gc_isadiabatic(const FieldC& F, const KArgs& A, 
     }
     const double g4 = 1.0 / __builtin_sqrt(1.0 - (v3 / C_LIGHT) * (v3 / C_LIGHT));
     const double cp = TWO_PI * g4 * mass / Bmag / __builtin_fabs(charge);
-    ok = cp / timescale<M>(F, t, y[0], y[1], y[2]) < A.epst;
+    ok = cp / timescale<M>(F, t, y[0], y[1], y[2]) < epst;
   }
   return ok;
 }

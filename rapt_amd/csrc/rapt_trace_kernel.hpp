@@ -412,7 +412,7 @@ void k_trace(KArgs A) {
             if (!gcmode) {
               sw = particle_isadiabatic<M>(F, A, xend, y, mass, charge);
             } else {
-              sw = !gc_isadiabatic<M>(F, A, x, y, mass, charge, mu);
+              sw = !gc_isadiabatic<M>(F, A.epss, A.epst, x, y, mass, charge, mu);
             }
           }
           if (sw && KIND != KIND_ADAPTIVE) {
