@@ -397,55 +397,71 @@ __global__ void k_to_bf16(const float* theta, bf16* wb, int64_t n) {
 // One kernel runs the whole ensemble forward pass, the disagreement test and
 // the baseline for a batch of queries; activations never leave registers.
 //
-// v_mfma_f32_16x16x32_bf16, 16 queries per wave: lane l holds query l%16 and
-// quarter h = l/16 of each activation.  A 16x16 output tile t leaves register
-// r of lane l holding neuron 16t + 4h + r (the D layout of the instruction:
-// D[4(l/16)+r][l%16]); tiles 2kb and 2kb+1 form the B operand of K-block kb
-// of the next layer, so K index 8h + j of block kb is input neuron
-// pin16(kb, h, j) = 32kb + 16(j/4) + 4h + j%4, and the packed weights are
-// P[t][kb][lane][j] = W[16t + lane%16][pin16(kb, lane/16, j)].  A block of
-// IW waves stages one member's packed 256x256 layer (128 KB) in LDS and its
-// waves (IW x 16 queries) read the A fragments from there, so L2 supplies
-// each weight once per block instead of once per wave.  Measured on the
-// 2^24-query four-member ensemble: 42.6 ms, against 45.8 ms for 32 queries
-// per wave with fragments read from L2 and 43.7 ms for 32 queries per wave
-// with LDS staging; 16 waves per block took 48.1 ms.
+// v_mfma_f32_32x32x16_bf16, 32 queries per wave: lane l holds query l%32 and
+// half h = l/32 of each activation.  A 32x32 output tile t leaves register r
+// of lane l holding neuron 32t + row32(r, h), row32(r, h) = 8(r/4) + 4h + r%4
+// (the D layout of the instruction: D[8(r/4)+4(l/32)+r%4][l%32]; A operand:
+// lane l holds row l%32, K indices 8(l/32)+j; B operand: column l%32, K
+// indices 8(l/32)+j).  Registers 8g..8g+7 of tile t form the B operand of
+// K-block kb = 2t+g of the next layer, so K index 8h + j of block kb is input
+// neuron pin32(kb, h, j) = 32(kb/2) + 16(kb%2) + 8(j/4) + 4h + j%4, and the
+// packed weights are P[t][kb][lane][j] = W[32t + lane%32][pin32(kb, lane/32,
+// j)].  A block of IW waves stages one member's packed 256x256 layer (128
+// KB) in LDS and its waves (IW x 32 queries) read the A fragments from
+// there, so L2 supplies each weight once per block instead of once per wave;
+// each fragment read from LDS feeds a 32-query MFMA.
 constexpr int MAXM = 4;   // largest ensemble
 
 struct InferArgs {
   int64_t n;
   int members;
-  const float* query;     // [n][3]
   const bf16* wb;         // [members][NPAD]
   const bf16* packed;     // [members][2][PACKED]: layers 2 and 3
   const bf16* p1;         // [members][16*64*8]: layer-1 A operands
   const float* norm;      // shift[3], scale[3]
-  double mass, eV;
   float thresh;           // relative member deviation that sends a query to the fallback
-  SplitC split;           // regime of the drift label (REGIME_SHIFT)
+  const float* prep;      // [n][8] k_prep's features and baselines
   float* out;             // [n][3] physical outputs (T_b, |omega_d|, latitude)
   float* raw;             // optional [members][n][3] normalised outputs
-  int32_t* flag;          // [n] in: k_regime's side | near << 1 (split on); out: 1 = fallback
+  int32_t* flag;          // [n] in: k_prep's side | near << 1; out: 1 = fallback
 };
 
-// one member's small parameters, staged in LDS: b1, b2, (b3, w4[0..2]), b4
+// one member's small parameters, staged in LDS: b1, b2, b3, w4 (one float4
+// of the three output weights per layer-3 neuron), b4
 struct SmallW {
-  float b1[H], b2[H];
-  float4 l3[H];
+  float b1[H], b2[H], b3[H];
+  float4 w4[H];
   float b4[4];
 };
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ int pin16(int kb, int h, int j) {
-  return 32 * kb + 16 * (j >> 2) + 4 * h + (j & 3);
+// the biases of one 32x32 output tile in a lane's accumulator order: register
+// r holds neuron row32(r, h), so b points at the tile's bias + 4h and the 16
+// values are four float4 runs at 8-element strides
+__device__ __forceinline__ f32x16 bias16(const float* b) {
+  f32x16 acc;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const float4 v = *(const float4*)(b + 8 * q);
+    acc[4 * q] = v.x;
+    acc[4 * q + 1] = v.y;
+    acc[4 * q + 2] = v.z;
+    acc[4 * q + 3] = v.w;
+  }
+  return acc;
 }
+
+__device__ __forceinline__ int pin32(int kb, int h, int j) {
+  return 32 * (kb >> 1) + 16 * (kb & 1) + 8 * (j >> 2) + 4 * h + (j & 3);
+}
+__device__ __forceinline__ int row32(int r, int h) { return 8 * (r >> 2) + 4 * h + (r & 3); }
 
 __global__ void k_pack(const bf16* wb, bf16* packed, int woff) {
   const int mem = blockIdx.z;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= PACKED) return;
-  const int j = idx & 7, lane = (idx >> 3) & 63, kb = (idx >> 9) & 7, tp = idx >> 12;
-  const int row = 16 * tp + (lane & 15), col = pin16(kb, lane >> 4, j);
+  const int j = idx & 7, lane = (idx >> 3) & 63, kb = (idx >> 9) & 15, tp = idx >> 13;
+  const int row = 32 * tp + (lane & 31), col = pin32(kb, lane >> 5, j);
   packed[(int64_t)mem * 2 * PACKED + idx] = wb[(int64_t)mem * NPAD + woff + row * H + col];
 }
 
@@ -453,50 +469,107 @@ __global__ void k_pack(const bf16* wb, bf16* packed, int woff) {
 // (x = hi + lo to 2^-17 relative), so K index j < 3 multiplies W1[.][j] by
 // hi_j and 3 <= j < 6 multiplies W1[.][j-3] by lo_{j-3}: the product with the
 // bf16 weights is as accurate as the FP32 vector form.  A operand of tile t
-// (16 neurons), lane (h, j): W1[16t + l%16][j%3] for h = 0, j < 6, else 0.
+// (32 neurons), lane (h, j): W1[32t + l%32][j%3] for h = 0, j < 6, else 0.
+constexpr int P1SIZE = 8 * 64 * 8;  // bf16 elements of one member's layer-1 operands
 __global__ void k_pack1(const bf16* wb, bf16* p1) {
   const int mem = blockIdx.z;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // 16 tiles * 64 lanes * 8
-  if (idx >= 16 * 64 * 8) return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // 8 tiles * 64 lanes * 8
+  if (idx >= P1SIZE) return;
   const int j = idx & 7, lane = (idx >> 3) & 63, t = idx >> 9;
-  const int n = 16 * t + (lane & 15);
+  const int n = 32 * t + (lane & 31);
   bf16 v = (bf16)0.0f;
-  if ((lane >> 4) == 0 && j < 6) v = wb[(int64_t)mem * NPAD + OFF_W1 + n * 3 + (j % 3)];
-  p1[(int64_t)mem * 16 * 64 * 8 + idx] = v;
+  if ((lane >> 5) == 0 && j < 6) v = wb[(int64_t)mem * NPAD + OFF_W1 + n * 3 + (j % 3)];
+  p1[(int64_t)mem * P1SIZE + idx] = v;
 }
 
-// Regime pre-pass for the inference kernel: flag[i] = side (bit 0) and
-// near (bit 1) of query i (regime()), so the double-precision field
-// evaluations stay out of the inference kernel's register budget.
-__global__ void k_regime(int64_t n, const float* query, SplitC S, int32_t* flag) {
+// Per-query pre-pass of the inference kernel: the regime flag (side, bit 0;
+// near, bit 1; 0 when the split is off), the three input features and the
+// three dipole baselines, prep[8i .. 8i+5] = (x0, x1, x2, lnT, lnW, lnLat).
+// The double-precision work (field evaluations, Walt's forms, the
+// mirror-latitude Newton loop) runs here on every lane of a wave, instead of
+// on the 16 query lanes of each inference wave inside its register budget.
+__global__ void k_prep(int64_t n, const float* query, SplitC S, double mass, double eV,
+                       float* prep, int32_t* flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float q[3] = {query[3 * i], query[3 * i + 1], query[3 * i + 2]};
   bool near;
-  const int side = regime(S, query[3 * i], query[3 * i + 1], &near);
+  const int side = regime(S, q[0], q[1], &near);
   flag[i] = side | (near ? 2 : 0);
+  float x[3];
+  features(q, x, side);
+  const Base b = baseline(q[0], q[1], q[2], mass, eV);
+  float4* o = (float4*)(prep + 8 * i);
+  o[0] = make_float4(x[0], x[1], x[2], b.lnT);
+  o[1] = make_float4(b.lnW, b.lnLat, 0.0f, 0.0f);
 }
 
-// IW = 8 waves per block (two per SIMD), 16 queries per wave.  The biases
-// start the accumulators.  Measured alternatives on the 2^24-query launch:
-// 2 or 4 query groups per wave (each A fragment from LDS feeding 2 or 4
-// MFMAs) spill 61-382 registers and took 51-73 ms against 44.7 ms.
+// IW = 8 waves per block (two per SIMD), 32 queries per wave.  The biases
+// start the accumulators.  Staging is pipelined: each member's two 256x256
+// layers pass through LDS as four 64 KB halves (layer 2 tiles 0-3, 4-7, then
+// layer 3), alternating between two buffers; while the waves compute with
+// one half, the next half (the next member's, or member 0's for the next
+// chunk) streams into the other buffer with direct global-to-LDS loads, so
+// the loads never wait on a barrier with the matrix cores idle.  A half is
+// 64 fragments of 1 KB; wave w loads fragments 8i + w.
 constexpr int IW = 8;
+constexpr int QW = 32;  // queries per wave
+constexpr int HALF = PACKED / 16;  // bf16x8 elements of half a packed layer (64 KB)
+
+// stage s of a member's four: layer 2 + (s >> 1), fragments 64 (s & 1) onward
+__device__ __forceinline__ void stage_load(const bf16* packed, int m, int s, bf16x8* buf,
+                                           int wv, int l) {
+  const bf16x8* src = (const bf16x8*)(packed + ((int64_t)m * 2 + (s >> 1)) * PACKED) +
+                      (s & 1) * HALF;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int f = 8 * i + wv;
+    __builtin_amdgcn_global_load_lds(
+        (const __attribute__((address_space(1))) void*)(src + 64 * f + l),
+        (__attribute__((address_space(3))) void*)(buf + 64 * f), 16, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0); }
+
+// one 32x32 output tile of a hidden layer from the fragments of buf (tile
+// t of the half: fragments 16 t .. 16 t + 15)
+__device__ __forceinline__ f32x16 tile32(const bf16x8* buf, int t, const bf16x8* in,
+                                         f32x16 acc, int l) {
+#pragma unroll
+  for (int kb = 0; kb < 16; ++kb)
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(buf[(16 * t + kb) * 64 + l], in[kb], acc, 0,
+                                                   0, 0);
+  return acc;
+}
+
 __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
-  __shared__ bf16x8 wst[PACKED / 8];
-  __shared__ SmallW sw;
-  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, h = l >> 4;
-  const int64_t groups = (A.n + 15) / 16;
+  __shared__ bf16x8 buf0[HALF], buf1[HALF];
+  __shared__ SmallW sw[MAXM];
+  const int l = threadIdx.x & 63, wv = threadIdx.x >> 6, h = l >> 5;
+  const int64_t groups = (A.n + QW - 1) / QW;
   const int64_t chunks = (groups + IW - 1) / IW;
+  if ((int64_t)blockIdx.x >= chunks) return;  // no work: no barrier is reached
+  for (int m = 0; m < A.members; ++m) {
+    const bf16* w = A.wb + (int64_t)m * NPAD;
+    for (int i = threadIdx.x; i < H; i += IW * 64) {
+      sw[m].b1[i] = (float)w[OFF_B1 + i];
+      sw[m].b2[i] = (float)w[OFF_B2 + i];
+      sw[m].b3[i] = (float)w[OFF_B3 + i];
+      sw[m].w4[i] = make_float4((float)w[OFF_W4 + i], (float)w[OFF_W4 + H + i],
+                                (float)w[OFF_W4 + 2 * H + i], 0.0f);
+    }
+    if (threadIdx.x < 3) sw[m].b4[threadIdx.x] = (float)w[OFF_B4 + threadIdx.x];
+  }
+  stage_load(A.packed, 0, 0, buf0, wv, l);
   for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
     const int64_t g = ch * IW + wv;
-    const int64_t qi = g * 16 + (l & 15);
+    const int64_t qi = g * QW + (l & 31);
     const bool valid = g < groups && qi < A.n;
-    float q[3], xin[3];
-    q[0] = valid ? A.query[qi * 3] : 5.0f;
-    q[1] = valid ? A.query[qi * 3 + 1] : 45.0f;
-    q[2] = valid ? A.query[qi * 3 + 2] : 5.0f;
-    const int rg = (valid && A.split.on) ? A.flag[qi] : 0;  // k_regime's side | near << 1
-    features(q, xin, rg & 1);
+    const float4* pq = (const float4*)(A.prep + 8 * (valid ? qi : 0));
+    const float4 p0 = pq[0];
+    const float xin[3] = {p0.x, p0.y, p0.z};
+    const int rg = valid ? A.flag[qi] : 0;  // k_prep's side | near << 1
     bf16x8 xb;
 #pragma unroll
     for (int j = 0; j < 8; ++j) xb[j] = (bf16)0.0f;
@@ -511,89 +584,80 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
     float mean[3] = {0.0f, 0.0f, 0.0f}, lo[3], hi[3];
     for (int c = 0; c < 3; ++c) { lo[c] = 3.0e38f; hi[c] = -3.0e38f; }
     for (int m = 0; m < A.members; ++m) {
-      const bf16* w = A.wb + (int64_t)m * NPAD;
-      const bf16x8* P1 = (const bf16x8*)(A.p1 + (int64_t)m * 16 * 64 * 8);
-      __syncthreads();
-      for (int i = threadIdx.x; i < H; i += IW * 64) {
-        sw.b1[i] = (float)w[OFF_B1 + i];
-        sw.b2[i] = (float)w[OFF_B2 + i];
-        sw.l3[i] = make_float4((float)w[OFF_B3 + i], (float)w[OFF_W4 + i],
-                               (float)w[OFF_W4 + H + i], (float)w[OFF_W4 + 2 * H + i]);
-      }
-      if (threadIdx.x < 3) sw.b4[threadIdx.x] = (float)w[OFF_B4 + threadIdx.x];
-      {
-        const bf16x8* src = (const bf16x8*)(A.packed + ((int64_t)m * 2) * PACKED);
-        for (int i = threadIdx.x; i < PACKED / 8; i += IW * 64) wst[i] = src[i];
-      }
-      __syncthreads();
-      // layer 1
-      bf16x8 act[8];
+      const SmallW& W = sw[m];
+      const bf16x8* P1 = (const bf16x8*)(A.p1 + (int64_t)m * P1SIZE);
+      const int mn = m + 1 < A.members ? m + 1 : 0;  // the member whose layer 2 comes next
+      // layer 1 (its operands come from global memory)
+      bf16x8 act[16];
+      wait_loads();
+      __syncthreads();  // first pass: sw written; every pass: this member's half 0 landed
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const float* bb = &sw.b1[16 * t + 4 * h];
-        f32x4 acc = {bb[0], bb[1], bb[2], bb[3]};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(P1[t * 64 + l], xb, acc, 0, 0, 0);
+      for (int t = 0; t < 8; ++t) {
+        f32x16 acc = bias16(&W.b1[32 * t + 4 * h]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P1[t * 64 + l], xb, acc, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) act[t >> 1][(t & 1) * 4 + r] = (bf16)silu(acc[r]);
+        for (int r = 0; r < 16; ++r) act[2 * t + (r >> 3)][r & 7] = (bf16)silu(acc[r]);
       }
-      // layer 2
-      bf16x8 nxt[8];
+      // layer 2, tiles 0-3 from buf0 while tiles 4-7 stream into buf1
+      stage_load(A.packed, m, 1, buf1, wv, l);
+      bf16x8 nxt[16];
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const float* bb = &sw.b2[16 * t + 4 * h];
-        f32x4 acc = {bb[0], bb[1], bb[2], bb[3]};
+      for (int t = 0; t < 4; ++t) {
+        f32x16 acc = tile32(buf0, t, act, bias16(&W.b2[32 * t + 4 * h]), l);
 #pragma unroll
-        for (int kb = 0; kb < 8; ++kb)
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wst[(t * 8 + kb) * 64 + l], act[kb], acc,
-                                                         0, 0, 0);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) nxt[t >> 1][(t & 1) * 4 + r] = (bf16)silu(acc[r]);
+        for (int r = 0; r < 16; ++r) nxt[2 * t + (r >> 3)][r & 7] = (bf16)silu(acc[r]);
       }
-      __syncthreads();
-      {
-        const bf16x8* src = (const bf16x8*)(A.packed + ((int64_t)m * 2 + 1) * PACKED);
-        for (int i = threadIdx.x; i < PACKED / 8; i += IW * 64) wst[i] = src[i];
+      wait_loads();
+      __syncthreads();  // buf1 landed; buf0 free
+      stage_load(A.packed, m, 2, buf0, wv, l);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        f32x16 acc = tile32(buf1, t, act, bias16(&W.b2[32 * (t + 4) + 4 * h]), l);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) nxt[2 * (t + 4) + (r >> 3)][r & 7] = (bf16)silu(acc[r]);
       }
-      __syncthreads();
+      wait_loads();
+      __syncthreads();  // layer 3 half 0 in buf0; buf1 free
+      stage_load(A.packed, m, 3, buf1, wv, l);
       // layer 3 and the output layer (FP32)
       float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
-#pragma unroll 2
-      for (int t = 0; t < 16; ++t) {
-        f32x4 acc;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[r] = sw.l3[16 * t + 4 * h + r].x;
+      for (int half = 0; half < 2; ++half) {
+        if (half == 1) {
+          wait_loads();
+          __syncthreads();  // layer 3 half 1 in buf1; buf0 free
+          stage_load(A.packed, mn, 0, buf0, wv, l);
+        }
 #pragma unroll
-        for (int kb = 0; kb < 8; ++kb)
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wst[(t * 8 + kb) * 64 + l], nxt[kb], acc,
-                                                         0, 0, 0);
+        for (int t = 0; t < 4; ++t) {
+          const int tt = 4 * half + t;
+          f32x16 acc = tile32(half ? buf1 : buf0, t, nxt, bias16(&W.b3[32 * tt + 4 * h]), l);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float4 c = sw.l3[16 * t + 4 * h + r];
-          const float v = silu(acc[r]);
-          // fused multiply-adds (the library compiles without contraction)
-          o0 = __builtin_fmaf(c.y, v, o0);
-          o1 = __builtin_fmaf(c.z, v, o1);
-          o2 = __builtin_fmaf(c.w, v, o2);
+          for (int r = 0; r < 16; ++r) {
+            const float4 c = W.w4[32 * tt + row32(r, h)];
+            const float v = silu(acc[r]);
+            // fused multiply-adds (the library compiles without contraction)
+            o0 = __builtin_fmaf(c.x, v, o0);
+            o1 = __builtin_fmaf(c.y, v, o1);
+            o2 = __builtin_fmaf(c.z, v, o2);
+          }
         }
       }
-      o0 += __shfl_xor(o0, 16);
-      o1 += __shfl_xor(o1, 16);
-      o2 += __shfl_xor(o2, 16);
       o0 += __shfl_xor(o0, 32);
       o1 += __shfl_xor(o1, 32);
       o2 += __shfl_xor(o2, 32);
-      const float o[3] = {o0 + sw.b4[0], o1 + sw.b4[1], o2 + sw.b4[2]};
+      const float o[3] = {o0 + W.b4[0], o1 + W.b4[1], o2 + W.b4[2]};
       for (int c = 0; c < 3; ++c) {
         mean[c] += o[c];
         lo[c] = fminf(lo[c], o[c]);
         hi[c] = fmaxf(hi[c], o[c]);
       }
-      if (A.raw && valid && l < 16)
+      if (A.raw && valid && l < 32)
         for (int c = 0; c < 3; ++c) A.raw[((int64_t)m * A.n + qi) * 3 + c] = o[c];
     }
-    if (valid && l < 16) {
-      const Base b = baseline(q[0], q[1], q[2], A.mass, A.eV);
-      const float bb[3] = {b.lnT, b.lnW, b.lnLat};
+    if (valid && l < 32) {
+      const float4 p1 = pq[1];
+      const float bb[3] = {p0.w, p1.x, p1.y};
       int bad = (rg & 2) ? 1 : 0;
       for (int k = 0; k < 3; ++k) {
         const float sc = A.norm[3 + k];
@@ -604,6 +668,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
       A.flag[qi] = bad;
     }
   }
+  wait_loads();  // the prefetch issued for a chunk this block does not run
 }
 
 // targets: normalised log-residuals from traced labels (T_b, omega_d, lat)
@@ -767,7 +832,7 @@ int rapt_mlp_create(rapt_ctx* c, int32_t members, const float* theta0, double ma
             hipMalloc(&m->m2, sizeof(float) * np) == hipSuccess &&
             hipMalloc(&m->wb, sizeof(bf16) * np) == hipSuccess &&
             hipMalloc(&m->packed, sizeof(bf16) * members * 2 * PACKED) == hipSuccess &&
-            hipMalloc(&m->p1, sizeof(bf16) * members * 16 * 64 * 8) == hipSuccess &&
+            hipMalloc(&m->p1, sizeof(bf16) * members * P1SIZE) == hipSuccess &&
             hipMalloc(&m->norm, sizeof(float) * 6) == hipSuccess;
   if (!ok) {
     rapt_mlp_destroy(m);
@@ -1004,15 +1069,16 @@ int rapt_mlp_infer(rapt_mlp* m, int64_t n, const float* query, float thresh, flo
     for (int layer = 0; layer < 2; ++layer)
       hipLaunchKernelGGL(k_pack, dim3(PACKED / 256, 1, m->members), dim3(256), 0, st, m->wb,
                          m->packed + (int64_t)layer * PACKED, layer == 0 ? OFF_W2 : OFF_W3);
-    hipLaunchKernelGGL(k_pack1, dim3(16 * 64 * 8 / 256, 1, m->members), dim3(256), 0, st, m->wb,
+    hipLaunchKernelGGL(k_pack1, dim3(P1SIZE / 256, 1, m->members), dim3(256), 0, st, m->wb,
                        m->p1);
     HIP_OK(c, hipGetLastError());
     m->packed_ok = true;
   }
-  float *dq = nullptr, *dout = nullptr, *draw = nullptr;
+  float *dq = nullptr, *dout = nullptr, *draw = nullptr, *dprep = nullptr;
   int32_t* dflag = nullptr;
   int rc = RAPT_OK;
   if (hipMalloc(&dq, sizeof(float) * 3 * n) != hipSuccess ||
+      hipMalloc(&dprep, sizeof(float) * 8 * n) != hipSuccess ||
       hipMalloc(&dout, sizeof(float) * 3 * n) != hipSuccess ||
       hipMalloc(&dflag, sizeof(int32_t) * n) != hipSuccess ||
       (raw && hipMalloc(&draw, sizeof(float) * 3 * n * m->members) != hipSuccess)) {
@@ -1024,28 +1090,24 @@ int rapt_mlp_infer(rapt_mlp* m, int64_t n, const float* query, float thresh, flo
     InferArgs A;
     A.n = n;
     A.members = m->members;
-    A.query = dq;
     A.wb = m->wb;
     A.packed = m->packed;
     A.p1 = m->p1;
     A.norm = m->norm;
-    A.mass = m->mass;
-    A.eV = m->eV;
     A.thresh = thresh;
-    A.split = m->split;
+    A.prep = dprep;
     A.out = dout;
     A.raw = draw;
     A.flag = dflag;
     int dev_cus = 256;
     (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, c->device);
-    // persistent: one block per CU, each taking chunks of IW x 16 queries
-    const int64_t groups = (n + 15) / 16;
+    // persistent: one block per CU, each taking chunks of IW x QW queries
+    const int64_t groups = (n + QW - 1) / QW;
     int64_t blocks = (groups + IW - 1) / IW;
     if (blocks > dev_cus) blocks = dev_cus;
     HIP_OK(c, hipEventRecord(c->ev0, st));
-    if (m->split.on)
-      hipLaunchKernelGGL(k_regime, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, dq,
-                         m->split, dflag);
+    hipLaunchKernelGGL(k_prep, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, dq,
+                       m->split, m->mass, m->eV, dprep, dflag);
     hipLaunchKernelGGL(k_infer, dim3((unsigned)blocks), dim3(IW * 64), 0, st, A);
     HIP_OK(c, hipEventRecord(c->ev1, st));
     if (hipGetLastError() != hipSuccess ||
@@ -1064,6 +1126,7 @@ int rapt_mlp_infer(rapt_mlp* m, int64_t n, const float* query, float thresh, flo
     }
   }
   (void)hipFree(dq);
+  (void)hipFree(dprep);
   (void)hipFree(dout);
   (void)hipFree(dflag);
   (void)hipFree(draw);
