@@ -159,6 +159,8 @@ def test_surrogate_training_and_fallback():
         t_tr = time.perf_counter() - t0
         print("65,536 queries: surrogate %.3f ms, traced %.1f ms, gain %.0fx" % (
             ms_inf, t_tr * 1e3, t_tr * 1e3 / ms_inf))
-        assert t_tr * 1e3 / ms_inf > 50
+        # measured: 0.115 ms against 67 ms (582x); the bar leaves half of it
+        # for run-to-run variation of the traced timing
+        assert t_tr * 1e3 / ms_inf > 300
     finally:
         s.close()
