@@ -561,6 +561,26 @@ def run_surrogate(args):
         rng = np.random.default_rng(seed)
         return np.c_[rng.uniform(2, 8, n), rng.uniform(10, 89, n), rng.uniform(4, 7, n)]
 
+    if args.infer_only:
+        # the inference kernels alone (their work does not depend on the
+        # weights): an untrained ensemble, 2^24 queries, for profiling
+        qin = qs(1 << 24, 204)
+        s = SG.Surrogate(f, members=4, seed=0)
+        s.infer(qin[:1 << 16])
+        kms = []
+        for i in range(max(args.steps, 1)):
+            kms.append(s.infer(qin)[3])
+            log("inference %d: %.2f ms" % (i, kms[-1]))
+        s.close()
+        ms = float(np.mean(kms))
+        res = base_line("surrogate inference kernel time, untrained ensemble (profiling)",
+                        len(qin) / (ms / 1e3), "queries/s", args, ms / 1e3 * len(kms),
+                        "bf16 MFMA / fp32 accumulate",
+                        {"workload": "MLP ensemble 4 x 3-256-256-256-3 SiLU, 2^24 queries, "
+                                     "untrained weights", "queries": len(qin)})
+        res["ms_per_step"] = ms
+        res["inference_kernel_ms"] = kms
+        return res
     ntr = args.n or (1 << 20)
     qtr, qte, qin = qs(ntr, 4), qs(1 << 16, 104), qs(1 << 24, 204)
     t0 = time.perf_counter()
@@ -651,6 +671,8 @@ def main():
                              "mixed_precision", "surrogate", "grid"))
     ap.add_argument("--epochs", type=int, default=100, help="surrogate training epochs")
     ap.add_argument("--dump", default="", help="surrogate: write held-out results to this .npz")
+    ap.add_argument("--infer-only", action="store_true",
+                    help="surrogate: time only the inference kernels (untrained ensemble)")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=0)
