@@ -40,21 +40,6 @@
 
 namespace rapt {
 
-// The launch's kernel-argument block, re-addressed at a point of use.  Both
-// kernels below take KArgs by value as their first argument, so it starts at
-// the kernarg segment pointer.  Arguments needed only when a particle
-// starts, finishes or writes a row are read through this pointer, made
-// opaque by an empty asm statement, so each load is issued where it is used.
-// Read through the by-value argument, every such load is hoisted to the
-// kernel's entry and all of them stay in scalar registers for the whole
-// launch, which spills them into VGPR lanes and reloads them with v_readlane
-// inside the step loop.
-using KArgsK = const __attribute__((address_space(4))) KArgs;
-__device__ __forceinline__ KArgsK* cold() {
-  KArgsK* p = (KArgsK*)__builtin_amdgcn_kernarg_segment_ptr();
-  asm volatile("" : "+s"(p));
-  return p;
-}
 
 // quad_perm control that reads lane I of this lane's group of L lanes
 // (2 bits per destination lane, lane 0 in bits 0-1)

@@ -9,8 +9,9 @@ namespace rapt {
 // ---- adiabaticity tests ------------------------------------------------------------
 // Particle.isadiabatic (rapt/Particle.py:345-384) on the row (t, r, p)
 template <int M>
-__device__ __forceinline__ bool particle_isadiabatic(const FieldC& F, const KArgs& A, double t, const double* y,
-                                     double mass, double charge) {
+__device__ __forceinline__ bool particle_isadiabatic(const FieldC& F, double epss, double epst,
+                                                     double t, const double* y, double mass,
+                                                     double charge) {
   const V3 p{y[3], y[4], y[5]};
   const double gm = __builtin_sqrt(mass * mass + dot3(p, p) / C_LIGHT_SQ);
   const V3 v{p.x / gm, p.y / gm, p.z / gm};
@@ -21,10 +22,10 @@ __device__ __forceinline__ bool particle_isadiabatic(const FieldC& F, const KArg
   const double vpar = dot3(v, B) / Bmag;
   const double vperp = __builtin_sqrt(vsq - vpar * vpar);
   const double rc = gamma * mass * vperp / (__builtin_fabs(charge) * Bmag);
-  bool ok = rc / lengthscale<M>(F, t, y[0], y[1], y[2]) < A.epss;
+  bool ok = rc / lengthscale<M>(F, t, y[0], y[1], y[2]) < epss;
   if (!F.is_static && ok) {
     const double cp = TWO_PI * gamma * mass / Bmag / __builtin_fabs(charge);
-    ok = cp / timescale<M>(F, t, y[0], y[1], y[2]) < A.epst;
+    ok = cp / timescale<M>(F, t, y[0], y[1], y[2]) < epst;
   }
   return ok;
 }

@@ -93,8 +93,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(trace_waves
 void k_trace(KArgs A) {
   const int eom = SP ? (int)EOM_TCB : A.eom;
   const FieldC& F = A.F;
-  const int64_t n = A.n;
-  double* S = A.state;
+  const int64_t n = cold()->n;
   bool active = false, exhausted = false;
   int64_t pi = 0;
   // per-lane particle + integrator state
@@ -106,10 +105,10 @@ void k_trace(KArgs A) {
   int64_t nrow = 0, rowcap = 0, rowbase = 0;
   int64_t c_acc = 0, c_rej = 0, c_stp = 0, c_fcn = 0, c_sw = 0, p_acc = 0;
   int32_t st = RAPT_ST_PENDING;
-  const int W = A.width;
 
   auto write_row = [&](double trow) {
-    double* r = A.rows + (rowbase + nrow) * W;
+    const int W = cold()->width;
+    double* r = cold()->rows + (rowbase + nrow) * W;
     if constexpr (KIND == KIND_ADAPTIVE) {
       r[0] = gcmode ? 1.0 : 0.0;
       r[1] = trow;
@@ -134,7 +133,7 @@ void k_trace(KArgs A) {
   // start an integrate() call towards x + dt (rapt/Particle.py:305, rapt/GuidingCenter.py:453)
   auto begin_call = [&]() {
     xend = x + dt;
-    hmax = (A.max_step == 0.0) ? xend - x : A.max_step;
+    hmax = (cold()->max_step == 0.0) ? xend - x : cold()->max_step;
     lfo = LFO0;
     nstep = 0;
     nacc_call = 0;
@@ -145,7 +144,7 @@ void k_trace(KArgs A) {
   // start a new advance() of the current segment at its current state
   auto begin_advance = [&](double tdelta_end) {
     tend = tdelta_end;
-    h = A.first_step;
+    h = cold()->first_step;
     tcur = x;
     if (!gcmode) {
       const V3 p{y[3], y[4], y[5]};
@@ -153,16 +152,17 @@ void k_trace(KArgs A) {
       const V3 vel{p.x / gm, p.y / gm, p.z / gm};
       const double gamma = 1.0 / __builtin_sqrt(1.0 - dot3(vel, vel) / C_LIGHT_SQ);
       const double B = magB<M>(F, x, y[0], y[1], y[2]);
-      dt = TWO_PI * gamma * mass / B / __builtin_fabs(charge) / A.cycres;
+      dt = TWO_PI * gamma * mass / B / __builtin_fabs(charge) / cold()->cycres;
       stepper = (KIND == KIND_ADAPTIVE) ? RAPT_STEP_DOP853 : A.stepper;
     } else if constexpr (KIND == KIND_ADAPTIVE) {
-      dt = A.gcdt;
+      dt = cold()->gcdt;
       stepper = RAPT_STEP_DOPRI5;
     }
     k1ok = false;
   };
   auto finish = [&](int32_t code) {
     st = code;
+    double* S = cold()->state;
     S[0 * n + pi] = x;
 #pragma unroll
     for (int i = 0; i < 6; ++i) S[(1 + i) * n + pi] = y[i];
@@ -174,26 +174,27 @@ void k_trace(KArgs A) {
     S[RAPT_S_GM * n + pi] = gm;
     S[RAPT_S_TCUR * n + pi] = tcur;
     int32_t fl = FLAG_STARTED | (gcmode ? FLAG_GCMODE : 0) | (st != RAPT_ST_CAP ? FLAG_FINAL : 0);
-    A.flags[pi] = fl;
-    A.status[pi] = st;
-    A.nrows[pi] = nrow;
-    A.naccpt[pi] = p_acc;
+    cold()->flags[pi] = fl;
+    cold()->status[pi] = st;
+    cold()->nrows[pi] = nrow;
+    cold()->naccpt[pi] = p_acc;
     active = false;
   };
 
   for (;;) {
     // ---- refill: take the next particle ---------------------------------------
     if (!active && !exhausted) {
-      pi = (int64_t)atomicAdd(&A.work[0], 1ULL);
-      if (pi < n && A.order) pi = A.order[pi];
+      pi = (int64_t)atomicAdd(&cold()->work[0], 1ULL);
+      if (pi < n && cold()->order) pi = cold()->order[pi];
       if (pi >= n) {
         exhausted = true;
-      } else if (A.flags[pi] & FLAG_FINAL) {
+      } else if (cold()->flags[pi] & FLAG_FINAL) {
         // finished in an earlier launch: no rows this time, status kept
-        A.nrows[pi] = 0;
-        A.naccpt[pi] = 0;
+        cold()->nrows[pi] = 0;
+        cold()->naccpt[pi] = 0;
       } else {
         active = true;
+        const double* S = cold()->state;
         x = S[0 * n + pi];
 #pragma unroll
         for (int i = 0; i < 6; ++i) y[i] = S[(1 + i) * n + pi];
@@ -206,10 +207,10 @@ void k_trace(KArgs A) {
         tend = S[RAPT_S_TEND * n + pi];
         gm = S[RAPT_S_GM * n + pi];
         tcur = S[RAPT_S_TCUR * n + pi];
-        const int32_t fl = A.flags[pi];
+        const int32_t fl = cold()->flags[pi];
         nrow = 0;
-        rowcap = A.row_cap[pi];
-        rowbase = A.row_off[pi];
+        rowcap = cold()->row_cap[pi];
+        rowbase = cold()->row_off[pi];
         p_acc = 0;
         k1ok = false;
         if constexpr (KIND == KIND_ADAPTIVE) {
@@ -218,7 +219,7 @@ void k_trace(KArgs A) {
           if (fl & FLAG_AINIT) {
             // Adaptive.__init__ (rapt/Adaptive.py:96-104): particle state in
             gcmode = false;
-            if (particle_isadiabatic<M>(F, A, x, y, mass, charge)) {
+            if (particle_isadiabatic<M>(F, cold()->epss, cold()->epst, x, y, mass, charge)) {
               if (particle_to_gc<M>(F, x, y, mass, charge, mu, v)) {
                 gcmode = true;
               } else {
@@ -231,12 +232,12 @@ void k_trace(KArgs A) {
               continue;
             }
             write_row(x);
-            begin_advance(x + A.delta);
+            begin_advance(x + cold()->delta);
           } else if (!(fl & FLAG_STARTED)) {
-            begin_advance(x + A.delta);
+            begin_advance(x + cold()->delta);
           }
         } else {
-          if (!(fl & FLAG_STARTED)) begin_advance(x + A.delta);
+          if (!(fl & FLAG_STARTED)) begin_advance(x + cold()->delta);
         }
         if (!(x < tend)) {
           finish(RAPT_ST_DONE);
@@ -408,11 +409,11 @@ void k_trace(KArgs A) {
         if (!finite) {
           finish(RAPT_ST_NAN);
         } else {
-          if (KIND == KIND_ADAPTIVE || A.check) {
+          if (KIND == KIND_ADAPTIVE || cold()->check) {
             if (!gcmode) {
-              sw = particle_isadiabatic<M>(F, A, xend, y, mass, charge);
+              sw = particle_isadiabatic<M>(F, cold()->epss, cold()->epst, xend, y, mass, charge);
             } else {
-              sw = !gc_isadiabatic<M>(F, A.epss, A.epst, x, y, mass, charge, mu);
+              sw = !gc_isadiabatic<M>(F, cold()->epss, cold()->epst, x, y, mass, charge, mu);
             }
           }
           if (sw && KIND != KIND_ADAPTIVE) {
@@ -437,7 +438,7 @@ void k_trace(KArgs A) {
             } else {
               write_row(trow);
               // t = current.tcur = segment start; advance(delta - t)
-              begin_advance(trow + (A.delta - trow));
+              begin_advance(trow + (cold()->delta - trow));
               if (!(x < tend)) {
                 finish(RAPT_ST_DONE);
               } else if (nrow >= rowcap) {
@@ -460,11 +461,11 @@ void k_trace(KArgs A) {
     }
   }
   // lane counters -> global (one atomic per lane and counter)
-  atomicAdd(&A.work[1], (unsigned long long)c_acc);
-  atomicAdd(&A.work[2], (unsigned long long)c_rej);
-  atomicAdd(&A.work[3], (unsigned long long)c_stp);
-  atomicAdd(&A.work[4], (unsigned long long)c_fcn);
-  atomicAdd(&A.work[5], (unsigned long long)c_sw);
+  atomicAdd(&cold()->work[1], (unsigned long long)c_acc);
+  atomicAdd(&cold()->work[2], (unsigned long long)c_rej);
+  atomicAdd(&cold()->work[3], (unsigned long long)c_stp);
+  atomicAdd(&cold()->work[4], (unsigned long long)c_fcn);
+  atomicAdd(&cold()->work[5], (unsigned long long)c_sw);
 }
 
 // ---- batched field evaluation (the _Field methods) -------------------------------
