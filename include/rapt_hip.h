@@ -188,7 +188,10 @@ int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt,
  * instructions, E's the next ne (ne = 0: E = 0).  Opcodes: 0 register[d] =
  * consts[a], 1 a+b, 2 a-b, 3 a*b, 4 a/b, 5 -a, 6 pow(a,b), 7 sqrt, 8 exp,
  * 9 log, 10 sin, 11 cos, 12 tan, 13 tanh, 14 sinh, 15 cosh, 16 atan, 17 asin,
- * 18 acos, 19 fabs, 20 atan2(a,b), 21 a (move), 22 log10, 23 hypot(a,b).
+ * 18 acos, 19 fabs, 20 atan2(a,b), 21 a (move), 22 log10, 23 hypot(a,b),
+ * 24 maximum(a,b), 25 minimum(a,b) (numpy's: a NaN operand propagates, ties
+ * give b),
+ * 26 sign(a) (numpy's: sign(0) = 0).
  * At most RAPT_EXPR_MAXCODE instructions and RAPT_EXPR_MAXCONST constants;
  * the library copies both into device memory owned by the context.  Then
  * rapt_field_set with model RAPT_FIELD_EXPR (params unused) selects it.

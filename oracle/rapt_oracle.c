@@ -77,6 +77,9 @@ static void expr_run(const orc_field *f, int lo, int n, const double *tp, double
         case 21: v = a; break;
         case 22: v = log10(a); break;
         case 23: v = hypot(a, b); break;
+        case 24: v = (a > b || a != a) ? a : b; break; /* numpy maximum: ties give b */
+        case 25: v = (a < b || a != a) ? a : b; break; /* numpy minimum */
+        case 26: v = a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : (a == 0.0 ? 0.0 : a)); break;
         default: v = NAN; break;
         }
         r[c[1] & 31] = v;

@@ -42,7 +42,8 @@ enum FieldModel : int {
 // instructions, E's the next ne (ne = 0: E = 0).  Opcodes: 0 const (a =
 // constant index), 1 add, 2 sub, 3 mul, 4 div, 5 neg, 6 pow, 7 sqrt, 8 exp,
 // 9 log, 10 sin, 11 cos, 12 tan, 13 tanh, 14 sinh, 15 cosh, 16 atan, 17 asin,
-// 18 acos, 19 fabs, 20 atan2, 21 mov, 22 log10, 23 hypot.
+// 18 acos, 19 fabs, 20 atan2, 21 mov, 22 log10, 23 hypot, 24 maximum, 25 minimum,
+// 26 sign (numpy's: NaN propagates, sign(0) = 0).
 struct ExprC {
   const int32_t* code;
   const double* k;
@@ -303,6 +304,9 @@ __device__ __noinline__ V3 expr_eval(ExprC X, int lo, int n, double t, double x,
       case 21: v = a; break;
       case 22: v = log10(a); break;
       case 23: v = hypot(a, b); break;
+      case 24: v = (a > b || a != a) ? a : b; break;  // numpy maximum: a NaN propagates,
+      case 25: v = (a < b || a != a) ? a : b; break;  // ties give b (numpy minimum alike)
+      case 26: v = a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : (a == 0.0 ? 0.0 : a)); break;  // sign
       default: v = __builtin_nan(""); break;
     }
     r[d & (EXPR_NREG - 1)] = v;

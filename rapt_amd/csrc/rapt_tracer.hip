@@ -294,7 +294,7 @@ int rapt_field_expr_set(rapt_ctx* c, int32_t nb, int32_t ne, const int32_t* code
     const int32_t op = code[4 * i], d = code[4 * i + 1], a = code[4 * i + 2], b = code[4 * i + 3];
     const bool regs = d >= 4 && d < EXPR_NREG && a >= 0 && b >= 0 &&
                       (op == 0 ? a < nconst : a < EXPR_NREG) && b < EXPR_NREG;
-    if (op < 0 || op > 23 || !regs)
+    if (op < 0 || op > 26 || !regs)
       return set_err(c, RAPT_E_ARG, "field program: bad instruction %d (%d %d %d %d)", i, op, d,
                      a, b);
   }

@@ -5,8 +5,10 @@ examples/Creating new fields.ipynb) traced into a small program.
 ``trace(field)`` calls the subclass's own ``B(tpos)`` and ``E(tpos)`` once
 with symbolic ``tpos = [t, x, y, z]``.  Arithmetic (``+ - * / **``, unary
 minus, ``abs``, Python's ``pow``) and numpy's elementwise functions
-(``np.sqrt``, ``np.exp``, ``np.tanh``, ``np.arctan2``, ...) on those symbols,
-and on numpy arrays built from them, record an expression graph; Python
+(``np.sqrt``, ``np.exp``, ``np.tanh``, ``np.arctan2``, ``np.maximum``,
+``np.minimum``, ``np.clip``, ``np.sign``, ...) on those symbols, and on numpy
+arrays built from them (``np.dot``, ``np.cross``, ``np.linalg.norm``,
+``np.sum``), record an expression graph; Python
 numbers are constants, combined as Python combines them.  The graph becomes a
 register program (four int32 per instruction: opcode, destination, operand a,
 operand b) that the device kernels and the oracle evaluate in double
@@ -25,7 +27,7 @@ OPS = {
     "const": 0, "add": 1, "sub": 2, "mul": 3, "div": 4, "neg": 5, "pow": 6, "sqrt": 7,
     "exp": 8, "log": 9, "sin": 10, "cos": 11, "tan": 12, "tanh": 13, "sinh": 14, "cosh": 15,
     "arctan": 16, "arcsin": 17, "arccos": 18, "abs": 19, "arctan2": 20, "mov": 21,
-    "log10": 22, "hypot": 23,
+    "log10": 22, "hypot": 23, "maximum": 24, "minimum": 25, "sign": 26,
 }
 NREG = 32          # registers of the device interpreter (0-3 hold t, x, y, z; 4-6 results)
 MAXCODE = 256      # instructions of both programs together
@@ -35,10 +37,11 @@ _NUMPY = {"add": np.add, "sub": np.subtract, "mul": np.multiply, "div": np.divid
           "pow": np.power, "arctan2": np.arctan2, "hypot": np.hypot, "neg": np.negative,
           "sqrt": np.sqrt, "exp": np.exp, "log": np.log, "log10": np.log10, "sin": np.sin,
           "cos": np.cos, "tan": np.tan, "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh,
-          "arctan": np.arctan, "arcsin": np.arcsin, "arccos": np.arccos, "abs": np.abs}
+          "arctan": np.arctan, "arcsin": np.arcsin, "arccos": np.arccos, "abs": np.abs,
+          "maximum": np.maximum, "minimum": np.minimum, "sign": np.sign}
 _UFUNC = {"true_divide": "div", "divide": "div", "add": "add", "subtract": "sub",
           "multiply": "mul", "power": "pow", "negative": "neg", "absolute": "abs",
-          "fabs": "abs"}
+          "fabs": "abs", "clip": "clip"}
 
 
 def _num(v):
@@ -48,6 +51,8 @@ def _num(v):
 def _apply(name, *args):
     """One operation: on numbers it is evaluated as numpy would (float64);
     otherwise it becomes a node of the graph."""
+    if name == "clip":  # np.clip(a, lo, hi) is minimum(maximum(a, lo), hi) elementwise
+        return _apply("minimum", _apply("maximum", args[0], args[1]), args[2])
     if all(_num(a) for a in args):
         return float(_NUMPY[name](*[np.float64(a) for a in args]))
     for a in args:
@@ -93,9 +98,16 @@ class Sym:
     def arccos(self): return Sym("arccos", (self,))
     def conjugate(self): return self
 
+    def clip(self, a_min=None, a_max=None, out=None, **kw):
+        """np.clip of one traced value (numpy calls the element's method)."""
+        v = self if a_min is None else _apply("maximum", self, a_min)
+        return v if a_max is None else _apply("minimum", v, a_max)
+
     def __bool__(self):
         raise NotImplementedError("a branch on a field value cannot run on the device: "
-                                  "write the field without value-dependent if/else")
+                                  "write the field without value-dependent if/else, "
+                                  "max() or min() (np.maximum, np.minimum, np.clip and "
+                                  "np.sign are traced)")
 
     def __float__(self):
         raise NotImplementedError("a field value converted to a Python float cannot be traced")
@@ -107,7 +119,7 @@ class Sym:
         if method != "__call__" or kw:
             raise NotImplementedError("numpy %s.%s cannot be traced" % (ufunc.__name__, method))
         name = _UFUNC.get(ufunc.__name__, ufunc.__name__)
-        if name not in _NUMPY:
+        if name not in _NUMPY and name != "clip":
             raise NotImplementedError("numpy function %s cannot be traced" % ufunc.__name__)
         if any(isinstance(v, np.ndarray) for v in inputs):
             f = np.frompyfunc(lambda *a: _apply(name, *a), len(inputs), 1)
@@ -227,9 +239,17 @@ def trace(field):
     """Program of a field whose B/E are Python code (the reference's override
     contract).  Raises NotImplementedError when they cannot be traced."""
     tpos = np.array([Sym("input", value=k) for k in range(4)], dtype=object)
-    with np.errstate(all="ignore"):
-        B = _components(type(field).B(field, tpos.copy()), "B")
-        E = _components(type(field).E(field, tpos.copy()), "E")
+    try:
+        with np.errstate(all="ignore"):
+            B = _components(type(field).B(field, tpos.copy()), "B")
+            E = _components(type(field).E(field, tpos.copy()), "E")
+    except NotImplementedError:
+        raise
+    except (TypeError, AttributeError, ValueError) as e:
+        # code that needs concrete numbers (a lookup, a library call
+        # outside numpy's elementwise math) fails on the symbols
+        raise NotImplementedError("the field's code needs concrete values (%s: %s)"
+                                  % (type(e).__name__, e)) from None
     consts, cidx = [], {}
     cb = _compile(B, consts, cidx)
     e_zero = all(n.op == "const" and n.value == 0.0 for n in E)
@@ -246,7 +266,10 @@ _MATH = {"add": lambda a, b: a + b, "sub": lambda a, b: a - b, "mul": lambda a, 
          "sqrt": lambda a: math.sqrt(a) if a >= 0.0 else math.nan, "exp": math.exp,
          "log": math.log, "log10": math.log10, "sin": math.sin, "cos": math.cos,
          "tan": math.tan, "tanh": math.tanh, "sinh": math.sinh, "cosh": math.cosh,
-         "arctan": math.atan, "arcsin": math.asin, "arccos": math.acos, "abs": abs}
+         "arctan": math.atan, "arcsin": math.asin, "arccos": math.acos, "abs": abs,
+         "maximum": lambda a, b: a if (a > b or a != a) else b,
+         "minimum": lambda a, b: a if (a < b or a != a) else b,
+         "sign": lambda a: 1.0 if a > 0 else (-1.0 if a < 0 else (0.0 if a == 0 else a))}
 
 
 def evaluate(prog, tpos):
@@ -264,7 +287,8 @@ def evaluate(prog, tpos):
                 r[d] = float(prog.consts[a])
             elif name == "mov":
                 r[d] = r[a]
-            elif name in ("add", "sub", "mul", "div", "pow", "arctan2", "hypot"):
+            elif name in ("add", "sub", "mul", "div", "pow", "arctan2", "hypot", "maximum",
+                          "minimum"):
                 r[d] = _MATH[name](r[a], r[b])
             else:
                 r[d] = _MATH[name](r[a])

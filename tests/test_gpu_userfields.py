@@ -189,3 +189,32 @@ def test_user_field_rejections(user_fields):
         batch.advance_particles(f, last, K.m_pr, K.e, 1.0, params=P(), precision="fp32_field")
     with pytest.raises(RuntimeError, match="dipole-type"):
         N.context().halfbounce(f.descriptor(), last[:, :4], np.array([1e-6]), 1e-3)
+
+
+def test_device_selection_ops():
+    """np.maximum / np.minimum / np.clip / np.sign, np.linalg.norm and np.cross
+    in a user field: device values against the host evaluation of the same
+    program (and numpy's special-value rules at +-0 and NaN)."""
+    from rapt_amd import fieldexpr as FE
+    from test_userfields import _SelectField
+
+    class M(rapt.fields._Field):
+        def B(self, tpos):
+            return np.array([np.maximum(tpos[1], tpos[2]), np.minimum(tpos[1], tpos[2]),
+                             np.sign(tpos[3])])
+    ctx = N.context()
+    for f, pts in ((_SelectField(), None), (M(), np.array(
+            [[0.0, np.nan, 1.0, 0.0], [0.0, 1.0, np.nan, -0.0], [0.0, -0.0, 0.0, np.nan],
+             [0.0, 0.0, -0.0, -2.0], [0.0, np.inf, -np.inf, 3.0]]))):
+        if pts is None:
+            rng = np.random.default_rng(22)
+            pts = np.c_[rng.uniform(0, 5, 256), rng.uniform(-3, 3, (256, 3)) * K.Re]
+            pts[:2, 2] = [0.0, -0.0]
+        d = f.descriptor()
+        got_b = ctx.field_eval(d, "B", pts)
+        got_e = ctx.field_eval(d, "E", pts)
+        for i, tp in enumerate(pts):
+            b, e = FE.evaluate(d["expr"], tp)
+            np.testing.assert_allclose(got_b[i], b, rtol=4e-16, atol=0)
+            np.testing.assert_allclose(got_e[i], e, rtol=4e-16, atol=0)
+            np.testing.assert_array_equal(np.signbit(got_b[i]), np.signbit(b))

@@ -192,3 +192,64 @@ def test_constant_and_zero_fields(user_fields):
     b, e = FE.evaluate(d["expr"], [1.0, 2.0, 3.0, 4.0])
     np.testing.assert_array_equal(b, [0.0, 0.0, 5e-8])
     np.testing.assert_array_equal(e, [0.0, 0.0, 0.0])
+
+
+class _SelectField(F._Field):
+    """A user field written with numpy's selection functions and vector
+    helpers: a saturating sheet (np.clip), a one-sided normal component
+    (np.maximum / np.minimum), a sign flip across y = 0, |r| from
+    np.linalg.norm and an E from np.cross."""
+
+    def __init__(self):
+        F._Field.__init__(self)
+        self.static = False
+        self.gradientstepsize = K.Re / 1000
+
+    def B(self, tpos):
+        r = np.asarray(tpos[1:])
+        z = tpos[3] / (0.5 * K.Re)
+        bx = 20e-9 * np.clip(z, -1.0, 1.0)
+        bz = np.maximum(1e-9 * np.sign(tpos[2]), 2e-10) + np.minimum(tpos[1] / K.Re, 0.0) * 1e-10
+        return np.array([bx, 3e-10 * np.linalg.norm(r) / K.Re, bz])
+
+    def E(self, tpos):
+        return np.cross(np.asarray(tpos[1:]) / K.Re, np.array([0.0, 0.0, 1e-4]))
+
+
+def test_selection_ops_and_vector_helpers(oracle):
+    f = _SelectField()
+    prog = f.descriptor()["expr"]
+    rng = np.random.default_rng(21)
+    pts = np.c_[rng.uniform(0, 5, 64), rng.uniform(-3, 3, (64, 3)) * K.Re]
+    pts[:4, 2] = [0.0, -0.0, 1e-300, -1e-300]  # sign at and near zero
+    fo = oracle.field_from_desc(f.descriptor())
+    ob, oe = oracle.field_eval(fo, "B", pts), oracle.field_eval(fo, "E", pts)
+    for i, tp in enumerate(pts):
+        b, e = FE.evaluate(prog, tp)
+        hb, he = f.B(tp.copy()), f.E(tp.copy())
+        # np.linalg.norm of a float array goes through BLAS, which fuses the
+        # products into the sum; traced, they are separate: an ulp apart
+        np.testing.assert_array_equal(b[[0, 2]], hb[[0, 2]])
+        np.testing.assert_allclose(b[1], hb[1], rtol=4e-16, atol=0)
+        np.testing.assert_array_equal(e, he)
+        np.testing.assert_array_equal(ob[i], b)
+        np.testing.assert_array_equal(oe[i], e)
+
+
+def test_selection_ops_follow_numpy_on_special_values():
+    """maximum/minimum propagate a NaN from either side and return the second
+    operand on ties (so the sign of a zero result is the second zero's),
+    sign(+-0) = +0, as numpy's ufuncs do (the device and the oracle implement
+    the same tests)."""
+    class M(F._Field):
+        def B(self, tpos):
+            return np.array([np.maximum(tpos[1], tpos[2]), np.minimum(tpos[1], tpos[2]),
+                             np.sign(tpos[3])])
+    prog = M().descriptor()["expr"]
+    for x, y, z in [(np.nan, 1.0, 0.0), (1.0, np.nan, -0.0), (-0.0, 0.0, np.nan),
+                    (0.0, -0.0, -2.0), (np.inf, -np.inf, 3.0)]:
+        tp = np.array([0.0, x, y, z])
+        got = FE.evaluate(prog, tp)[0]
+        want = np.array([np.maximum(x, y), np.minimum(x, y), np.sign(z)])
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(np.signbit(got), np.signbit(want))
