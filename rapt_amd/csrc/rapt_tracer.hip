@@ -290,13 +290,28 @@ int rapt_field_expr_set(rapt_ctx* c, int32_t nb, int32_t ne, const int32_t* code
     return set_err(c, RAPT_E_ARG, "field program: %d + %d instructions, %d constants (at most "
                    "%d instructions, %d constants)", nb, ne, nconst, RAPT_EXPR_MAXCODE,
                    RAPT_EXPR_MAXCONST);
-  for (int32_t i = 0; i < nb + ne; ++i) {
-    const int32_t op = code[4 * i], d = code[4 * i + 1], a = code[4 * i + 2], b = code[4 * i + 3];
-    const bool regs = d >= 4 && d < EXPR_NREG && a >= 0 && b >= 0 &&
-                      (op == 0 ? a < nconst : a < EXPR_NREG) && b < EXPR_NREG;
-    if (op < 0 || op > 26 || !regs)
-      return set_err(c, RAPT_E_ARG, "field program: bad instruction %d (%d %d %d %d)", i, op, d,
-                     a, b);
+  // every operand register is an input (0-3) or written earlier in its own
+  // program, and each program writes its three results (registers 4-6), so
+  // the interpreter never reads a register it did not set
+  for (int prog = 0; prog < 2; ++prog) {
+    const int32_t lo = prog == 0 ? 0 : nb, hi = prog == 0 ? nb : nb + ne;
+    if (lo == hi) continue;
+    uint32_t set = 0xFu;
+    for (int32_t i = lo; i < hi; ++i) {
+      const int32_t op = code[4 * i], d = code[4 * i + 1], a = code[4 * i + 2],
+                    b = code[4 * i + 3];
+      const bool binary = op >= 1 && op <= 4 ? true : (op == 6 || op == 20 || op == 23 ||
+                                                      op == 24 || op == 25);
+      const bool regs = d >= 4 && d < EXPR_NREG && a >= 0 && b >= 0 &&
+                        (op == 0 ? a < nconst : a < EXPR_NREG) && b < EXPR_NREG;
+      const bool defined = op == 0 || (((set >> a) & 1u) && (!binary || ((set >> b) & 1u)));
+      if (op < 0 || op > 26 || !regs || !defined)
+        return set_err(c, RAPT_E_ARG, "field program: bad instruction %d (%d %d %d %d)", i, op,
+                       d, a, b);
+      set |= 1u << d;
+    }
+    if ((set & 0x70u) != 0x70u)
+      return set_err(c, RAPT_E_ARG, "field program %d leaves a result register unset", prog);
   }
   HIP_OK(c, hipSetDevice(c->device));
   const size_t kb = sizeof(double) * (size_t)(nconst > 0 ? nconst : 1);

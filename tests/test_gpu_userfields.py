@@ -218,3 +218,27 @@ def test_device_selection_ops():
             np.testing.assert_allclose(got_b[i], b, rtol=4e-16, atol=0)
             np.testing.assert_allclose(got_e[i], e, rtol=4e-16, atol=0)
             np.testing.assert_array_equal(np.signbit(got_b[i]), np.signbit(b))
+
+
+def test_malformed_programs_rejected():
+    """rapt_field_expr_set refuses a program that reads a register it did not
+    set, leaves a result register unset, or names a constant it lacks; the
+    error is raised before anything runs."""
+    import ctypes as C
+    ctx = N.context()
+    ok = np.array([[0, 4, 0, 0], [0, 5, 0, 0], [0, 6, 0, 0]], np.int32)
+    k = np.array([1.0])
+    cases = [np.array([[3, 7, 1, 9], [21, 4, 7, 0], [0, 5, 0, 0], [0, 6, 0, 0]], np.int32),
+             np.array([[0, 4, 0, 0], [0, 5, 0, 0]], np.int32),
+             np.array([[0, 4, 3, 0], [0, 5, 0, 0], [0, 6, 0, 0]], np.int32)]
+    L = N.lib()
+    assert L.rapt_field_expr_set(ctx.h, 3, 0, ok.ctypes.data_as(C.POINTER(C.c_int32)), 1,
+                                 k.ctypes.data_as(C.POINTER(C.c_double))) == 0
+    for code in cases:
+        rc = L.rapt_field_expr_set(ctx.h, len(code), 0,
+                                   code.ctypes.data_as(C.POINTER(C.c_int32)), 1,
+                                   k.ctypes.data_as(C.POINTER(C.c_double)))
+        assert rc == -1  # RAPT_E_ARG
+    # the context now holds this test's program: make the next field set upload again
+    ctx._expr_key = None
+    ctx._field_key = None
