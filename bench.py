@@ -689,6 +689,9 @@ def main():
     ap.add_argument("--top", type=int, default=0,
                     help="trace only this many of the most expensive particles (critical "
                          "path of the launch); implies --no-cpu")
+    ap.add_argument("--skip-top", type=int, default=0,
+                    help="trace all but this many of the most expensive particles (the "
+                         "batch without its critical path); implies --no-cpu")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the launch-shape autotuner (use --lpp/--lanes as given)")
     ap.add_argument("--cpu-sample", type=int, default=8192)
@@ -723,7 +726,8 @@ def main():
                "grid": run_grid, "mixed_precision": run_mixed_precision,
                "surrogate": run_surrogate, "gc_doubledipole": run_gc_doubledipole}
     res = runners[args.workload](args)
-    if args.workload == "gc_doubledipole" and not args.headline_only and not args.top:
+    if args.workload == "gc_doubledipole" and not args.headline_only and not args.top \
+            and not args.skip_top:
         res["configs"] = other_configs(args)
     print(json.dumps(res), flush=True)
 
@@ -777,17 +781,19 @@ def run_gc_doubledipole(args):
     desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes=args.lanes, block=args.block,
                        lanes_per_particle=args.lpp, work_order=1 if args.order == "input" else 0,
                        mixed_quads=args.mixed_quads, mixed_octets=args.mixed_octets)
-    if args.top:
+    if args.top or args.skip_top:
         # only the `top` most expensive particles (accepted steps of a full-advance
-        # trial): the launch's critical path without the rest of the batch
+        # trial): the launch's critical path without the rest of the batch; or
+        # all but the `skip_top` most expensive: the rest without its critical path
         from rapt_amd import autotune as AT
         full = AT.trial_cost(w["field"], S, flags, cap, desc, trial_fraction=1.0)
-        idx = np.argsort(-full, kind="stable")[:args.top]
+        rank = np.argsort(-full, kind="stable")
+        idx = rank[:args.top] if args.top else rank[args.skip_top:]
         S, flags, cap = S[:, idx], flags[idx], cap[idx]
         n = len(idx)
         args.no_cpu = True
-        log("top %d particles: %.0f-%.0f accepted steps each" % (n, full[idx].min(),
-                                                                 full[idx].max()))
+        log("%d particles: %.0f-%.0f accepted steps each" % (n, full[idx].min(),
+                                                             full[idx].max()))
     cost = None
     cost_s = None
     if args.order == "cost":
@@ -850,7 +856,7 @@ def run_gc_doubledipole(args):
         % (out["naccpt"].mean(), *q))
     # the library's default work order (shortest output step first), same shape
     dflt = None
-    if args.order == "cost" and not args.top:
+    if args.order == "cost" and not args.top and not args.skip_top:
         b2 = N.Batch(ctx, n)
         try:
             b2.upload(S, flags)
