@@ -295,7 +295,9 @@ int rapt_particle_gc(rapt_ctx *ctx, int64_t n, const double *rows, const double 
  * forms (Walt's bounce and drift periods, rapt/utils.py:435-513, and the
  * exact dipole mirror latitude).  Parameters per member: RAPT_MLP_NPARAM
  * floats, W1[256][3], b1[256], W2[256][256] (out, in), b2, W3, b3, W4[3][256],
- * b4[3]. */
+ * b4[3], in base-2 units: the plain SiLU network's parameters are
+ * (ln2 W1, ln2 b1, W2, ln2 b2, W3, ln2 b3, log2(e) W4, b4), and each hidden
+ * unit computes a / (1 + 2^-a) of its pre-activation a (= log2(e) silu(z)). */
 #define RAPT_MLP_NPARAM 133379
 typedef struct rapt_mlp rapt_mlp;
 

@@ -49,15 +49,23 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// logistic function from v_exp_f32 and v_rcp_f32 (about 1 ulp each); training
-// and inference use this same definition
-__device__ __forceinline__ float sigm(float z) {
-  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+// The network is a SiLU MLP whose parameters are kept in base-2 units.  With
+// the plain network's (W1, b1, W2, b2, W3, b3, W4, b4) the stored parameters
+// are (log2(e) W1, log2(e) b1, W2, log2(e) b2, W3, log2(e) b3, ln(2) W4, b4):
+// every hidden pre-activation is then a = log2(e) z and every hidden output
+// g = log2(e) silu(z) = a / (1 + 2^-a), the same function, while the logistic
+// factor is rcp(1 + exp2(-a)), one v_exp_f32 and one v_rcp_f32 (about 1 ulp
+// each) with no scaling multiply in front.  Training and inference use these
+// same definitions, so both round the same hidden values to bf16.
+constexpr float LN2 = 0.6931471805599453f;
+__device__ __forceinline__ float sig2(float a) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-a));
 }
-__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
-__device__ __forceinline__ float dsilu(float z) {
-  const float s = sigm(z);
-  return s * (1.0f + z * (1.0f - s));
+__device__ __forceinline__ float act2(float a) { return a * sig2(a); }
+// d act2 / da = s (1 + ln2 a (1 - s)), s = sig2(a)
+__device__ __forceinline__ float dact2(float a) {
+  const float s = sig2(a);
+  return s * (1.0f + LN2 * a * (1.0f - s));
 }
 
 // ---- dipole baselines (the residual targets' reference point) -----------------------
@@ -204,7 +212,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const bf16* A
   }
 }
 
-// layer 1: Z1 = W1 x + b1, H1 = silu(Z1) (bf16).  grid (Bt*256/256, 1, members)
+// layer 1: Z1 = W1 x + b1, H1 = act2(Z1) (bf16).  grid (Bt*256/256, 1, members)
 __global__ void k_in_fwd(int Bt, const float* X, const bf16* wb, float* Z1, bf16* H1,
                          int64_t sZ) {
   const int mem = blockIdx.z;
@@ -215,10 +223,10 @@ __global__ void k_in_fwd(int Bt, const float* X, const bf16* wb, float* Z1, bf16
   float z = (float)w[OFF_B1 + n];
   for (int i = 0; i < 3; ++i) z += (float)w[OFF_W1 + n * 3 + i] * X[s * 3 + i];
   Z1[mem * sZ + idx] = z;
-  H1[mem * sZ + idx] = (bf16)silu(z);
+  H1[mem * sZ + idx] = (bf16)act2(z);
 }
 
-// Z += b (layer bias at boff), H = silu(Z), as bf16 (Hb) or fp32 (Hf)
+// Z += b (layer bias at boff), H = act2(Z), as bf16 (Hb) or fp32 (Hf)
 __global__ void k_bias_act(int Bt, const bf16* wb, int boff, float* Z, bf16* Hb, float* Hf,
                            int64_t sZ) {
   const int mem = blockIdx.z;
@@ -227,8 +235,8 @@ __global__ void k_bias_act(int Bt, const bf16* wb, int boff, float* Z, bf16* Hb,
   const int n = (int)(idx % H);
   const float z = Z[mem * sZ + idx] + (float)wb[(int64_t)mem * NPAD + boff + n];
   Z[mem * sZ + idx] = z;
-  if (Hb) Hb[mem * sZ + idx] = (bf16)silu(z);
-  if (Hf) Hf[mem * sZ + idx] = silu(z);
+  if (Hb) Hb[mem * sZ + idx] = (bf16)act2(z);
+  if (Hf) Hf[mem * sZ + idx] = act2(z);
 }
 
 // output layer and loss: Y = W4 H3 + b4; dY = 2 (Y - T) / (3 Bt); loss sum
@@ -258,7 +266,7 @@ __global__ void k_out_fwd(int Bt, const float* H3, const bf16* wb, const float* 
   }
 }
 
-// dH3 = dY W4, dZ3 = dH3 silu'(Z3) -> bf16
+// dH3 = dY W4, dZ3 = dH3 act2'(Z3) -> bf16
 __global__ void k_out_bwd(int Bt, const float* dY, const bf16* wb, const float* Z3, bf16* dZ3,
                           int64_t sZ) {
   const int mem = blockIdx.z;
@@ -269,15 +277,15 @@ __global__ void k_out_bwd(int Bt, const float* dY, const bf16* wb, const float* 
   const float* dy = dY + (int64_t)mem * Bt * 3 + s * 3;
   const float g = dy[0] * (float)w[OFF_W4 + n] + dy[1] * (float)w[OFF_W4 + H + n] +
                   dy[2] * (float)w[OFF_W4 + 2 * H + n];
-  dZ3[mem * sZ + idx] = (bf16)(g * dsilu(Z3[mem * sZ + idx]));
+  dZ3[mem * sZ + idx] = (bf16)(g * dact2(Z3[mem * sZ + idx]));
 }
 
-// dZ = dH silu'(Z) -> bf16
+// dZ = dH act2'(Z) -> bf16
 __global__ void k_act_bwd(int Bt, const float* dH, const float* Z, bf16* dZ, int64_t sZ) {
   const int mem = blockIdx.z;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)Bt * H) return;
-  dZ[mem * sZ + idx] = (bf16)(dH[mem * sZ + idx] * dsilu(Z[mem * sZ + idx]));
+  dZ[mem * sZ + idx] = (bf16)(dH[mem * sZ + idx] * dact2(Z[mem * sZ + idx]));
 }
 
 // Batch reductions in two deterministic stages: per chunk of RCH samples
@@ -296,7 +304,7 @@ __global__ void k_colsum_part(int Bt, const bf16* dZ, float* part, int slot, int
 }
 
 // partials of dW4[k][n] = sum_s dY[s][k] H3[s][n] (slots 0..2), dW1[n][i] =
-// sum_s dZ1[s][n] X[s][i] (slots 3..5, dZ1 = dH1 silu'(Z1) in fp32), db1
+// sum_s dZ1[s][n] X[s][i] (slots 3..5, dZ1 = dH1 act2'(Z1) in fp32), db1
 // (slot 6) and db4 (slot 7, columns 0..2)
 __global__ void k_small_part(int Bt, const float* dY, const float* H3, const float* dH1,
                              const float* Z1, const float* X, float* part, int64_t sZ) {
@@ -310,7 +318,7 @@ __global__ void k_small_part(int Bt, const float* dY, const float* H3, const flo
   for (int s = c * RCH; s < (c + 1) * RCH; ++s) {
     const float hv = h3[(int64_t)s * H + n];
     for (int k = 0; k < 3; ++k) w4[k] += dy[s * 3 + k] * hv;
-    const float dz = dh1[(int64_t)s * H + n] * dsilu(z1[(int64_t)s * H + n]);
+    const float dz = dh1[(int64_t)s * H + n] * dact2(z1[(int64_t)s * H + n]);
     b1 += dz;
     for (int i = 0; i < 3; ++i) w1[i] += dz * X[s * 3 + i];
     if (n < 3) b4 += dy[s * 3 + n];
@@ -596,7 +604,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
         f32x16 acc = bias16(&W.b1[32 * t + 4 * h]);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P1[t * 64 + l], xb, acc, 0, 0, 0);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) act[2 * t + (r >> 3)][r & 7] = (bf16)silu(acc[r]);
+        for (int r = 0; r < 16; ++r) act[2 * t + (r >> 3)][r & 7] = (bf16)act2(acc[r]);
       }
       // layer 2, tiles 0-3 from buf0 while tiles 4-7 stream into buf1
       stage_load(A.packed, m, 1, buf1, wv, l);
@@ -605,7 +613,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
       for (int t = 0; t < 4; ++t) {
         f32x16 acc = tile32(buf0, t, act, bias16(&W.b2[32 * t + 4 * h]), l);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) nxt[2 * t + (r >> 3)][r & 7] = (bf16)silu(acc[r]);
+        for (int r = 0; r < 16; ++r) nxt[2 * t + (r >> 3)][r & 7] = (bf16)act2(acc[r]);
       }
       wait_loads();
       __syncthreads();  // buf1 landed; buf0 free
@@ -614,7 +622,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
       for (int t = 0; t < 4; ++t) {
         f32x16 acc = tile32(buf1, t, act, bias16(&W.b2[32 * (t + 4) + 4 * h]), l);
 #pragma unroll
-        for (int r = 0; r < 16; ++r) nxt[2 * (t + 4) + (r >> 3)][r & 7] = (bf16)silu(acc[r]);
+        for (int r = 0; r < 16; ++r) nxt[2 * (t + 4) + (r >> 3)][r & 7] = (bf16)act2(acc[r]);
       }
       wait_loads();
       __syncthreads();  // layer 3 half 0 in buf0; buf1 free
@@ -635,7 +643,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
             const float4 c = W.w4[32 * tt + row32(r, h)];
-            const float v = silu(acc[r]);
+            const float v = act2(acc[r]);
             // fused multiply-adds (the library compiles without contraction)
             o0 = __builtin_fmaf(c.x, v, o0);
             o1 = __builtin_fmaf(c.y, v, o1);

@@ -9,6 +9,9 @@ latitude [deg]).  Labels come from the traced bounce quantities on the GPU
 flutils.bounceperiod, rapt/flutils.py:232-252).  At inference, queries on
 which the members' relative spread exceeds a threshold are traced instead,
 so the traced method is both the fallback and the yardstick of the surrogate.
+Parameters are held in base-2 units (hidden units compute a / (1 + 2^-a) =
+log2(e) silu(z)); the plain SiLU network's are
+(ln2 W1, ln2 b1, W2, ln2 b2, W3, ln2 b3, log2(e) W4, b4).
 
 Training and inference run on the device (rapt_amd/csrc/rapt_mlp.hip); the
 host only shuffles sample indices and holds the arrays.

@@ -5,6 +5,9 @@
   missing SiLU' factor fails this).
 - bfloat16 rounding against exactly representable values and ties.
 - The rounded forward pass stays within bf16 precision of the unrounded one.
+- The base-2 parametrisation is the SiLU network: act2 at exact points, and
+  the unrounded forward against a plain SiLU network built from converted
+  parameters (a factor of ln 2 dropped or misplaced anywhere fails this).
 """
 import numpy as np
 
@@ -49,3 +52,27 @@ def test_rounded_forward_close_to_unrounded():
     y0, _ = O.forward(theta, X, quantize=False)
     y1, _ = O.forward(theta, X, quantize=True)
     assert np.max(np.abs(y1 - y0)) < 0.05 * np.max(np.abs(y0)) + 1e-3
+
+
+def test_base2_activation_known_values():
+    a = np.array([0.0, 1.0, -1.0, 2.0, -2.0])
+    # a / (1 + 2^-a): 0, 2/3, -1/3, 8/5, -2/5 exactly
+    np.testing.assert_allclose(O.act(a), [0.0, 2 / 3, -1 / 3, 8 / 5, -2 / 5], rtol=1e-15, atol=0)
+    z = np.linspace(-8, 8, 33)
+    np.testing.assert_allclose(O.act(z / np.log(2.0)), O.silu(z) / np.log(2.0), rtol=1e-14,
+                               atol=1e-300)
+    h = 1e-6
+    fd = (O.act(z + h) - O.act(z - h)) / (2 * h)
+    np.testing.assert_allclose(O.dact(z), fd, rtol=1e-8, atol=1e-10)
+
+
+def test_base2_network_is_the_silu_network():
+    rng = np.random.default_rng(4)
+    theta = O.init(7)[0].astype(np.float64)
+    for lo, hi in ((O.OFF_B1, O.OFF_W2), (O.OFF_B2, O.OFF_W3), (O.OFF_B3, O.OFF_W4),
+                   (O.OFF_B4, O.NPARAM)):
+        theta[lo:hi] = rng.normal(0, 0.3, hi - lo)
+    X = rng.uniform(-1.5, 1.5, (128, 3))
+    y2, _ = O.forward(theta, X, quantize=False)
+    ys = O.silu_forward(O.to_silu_params(theta), X)
+    assert np.max(np.abs(y2 - ys)) <= 1e-12 * np.max(np.abs(ys))
