@@ -524,30 +524,53 @@ constexpr int IW = 8;
 constexpr int QW = 32;  // queries per wave
 constexpr int HALF = PACKED / 16;  // bf16x8 elements of half a packed layer (64 KB)
 
-// stage s of a member's four: layer 2 + (s >> 1), fragments 64 (s & 1) onward
-__device__ __forceinline__ void stage_load(const bf16* packed, int m, int s, bf16x8* buf,
-                                           int wv, int l) {
-  const bf16x8* src = (const bf16x8*)(packed + ((int64_t)m * 2 + (s >> 1)) * PACKED) +
-                      (s & 1) * HALF;
+// stage s of a member's four: layer 2 + (s >> 1), fragments 64 (s & 1) onward.
+// Wave wv loads fragments 8i + wv, starting at i = rot (the block's rotation,
+// so that the CUs of an XCD do not all ask the L2 for the same lines at the
+// same moment).  The loads go through a buffer resource
+// (buffer_load_dwordx4 ... lds), which counts on vmcnt alone: with the FLAT
+// form (global_load_lds) in flight the compiler must treat the LDS counter as
+// unordered, and every LDS read of the half being computed would then wait
+// for all reads issued before it.
+__device__ __forceinline__ void stage_load(__amdgpu_buffer_rsrc_t rs, int m, int s, bf16x8* buf,
+                                           int wv, int l, int rot) {
+  const int base = (m * 2 + (s >> 1)) * PACKED * 2 + (s & 1) * HALF * 16;  // bytes
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const int f = 8 * i + wv;
-    __builtin_amdgcn_global_load_lds(
-        (const __attribute__((address_space(1))) void*)(src + 64 * f + l),
-        (__attribute__((address_space(3))) void*)(buf + 64 * f), 16, 0, 0);
+    const int f = 8 * ((i + rot) & 7) + wv;
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        rs, (__attribute__((address_space(3))) void*)(buf + 64 * f), 16,
+        base + (64 * f + l) * 16, 0, 0, 0);
   }
 }
 
 __device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0); }
 
 // one 32x32 output tile of a hidden layer from the fragments of buf (tile
-// t of the half: fragments 16 t .. 16 t + 15)
+// t of the half: fragments 16 t .. 16 t + 15).  The A fragments are read
+// PF ahead into a ring of registers, so each MFMA's LDS read has PF - 1
+// matrix operations of cover instead of none.
+constexpr int PF = 4;
 __device__ __forceinline__ f32x16 tile32(const bf16x8* buf, int t, const bf16x8* in,
                                          f32x16 acc, int l) {
+  const bf16x8* p = buf + 16 * t * 64 + l;
+  bf16x8 a[PF];
 #pragma unroll
-  for (int kb = 0; kb < 16; ++kb)
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(buf[(16 * t + kb) * 64 + l], in[kb], acc, 0,
-                                                   0, 0);
+  for (int i = 0; i < PF; ++i) a[i] = p[i * 64];
+#pragma unroll
+  for (int kb = 0; kb < 16; ++kb) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[kb % PF], in[kb], acc, 0, 0, 0);
+    if (kb + PF < 16) a[kb % PF] = p[(kb + PF) * 64];
+  }
+  // keep that order in the schedule (left alone, the scheduler sinks each
+  // read next to its MFMA to save registers)
+  __builtin_amdgcn_sched_group_barrier(0x100, PF, 0);
+#pragma unroll
+  for (int kb = 0; kb < 16; ++kb) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    if (kb + PF < 16) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+  }
   return acc;
 }
 
@@ -569,7 +592,11 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
     }
     if (threadIdx.x < 3) sw[m].b4[threadIdx.x] = (float)w[OFF_B4 + threadIdx.x];
   }
-  stage_load(A.packed, 0, 0, buf0, wv, l);
+  // the packed layers 2 and 3 of every member; reads past the end return 0
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)A.packed, (short)0, A.members * 2 * PACKED * (int)sizeof(bf16), 0x00020000);
+  const int rot = blockIdx.x & 7;
+  stage_load(rs, 0, 0, buf0, wv, l, rot);
   for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
     const int64_t g = ch * IW + wv;
     const int64_t qi = g * QW + (l & 31);
@@ -607,7 +634,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
         for (int r = 0; r < 16; ++r) act[2 * t + (r >> 3)][r & 7] = (bf16)act2(acc[r]);
       }
       // layer 2, tiles 0-3 from buf0 while tiles 4-7 stream into buf1
-      stage_load(A.packed, m, 1, buf1, wv, l);
+      stage_load(rs, m, 1, buf1, wv, l, rot);
       bf16x8 nxt[16];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -617,7 +644,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
       }
       wait_loads();
       __syncthreads();  // buf1 landed; buf0 free
-      stage_load(A.packed, m, 2, buf0, wv, l);
+      stage_load(rs, m, 2, buf0, wv, l, rot);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         f32x16 acc = tile32(buf1, t, act, bias16(&W.b2[32 * (t + 4) + 4 * h]), l);
@@ -626,7 +653,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
       }
       wait_loads();
       __syncthreads();  // layer 3 half 0 in buf0; buf1 free
-      stage_load(A.packed, m, 3, buf1, wv, l);
+      stage_load(rs, m, 3, buf1, wv, l, rot);
       // layer 3 and the output layer (FP32)
       float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
 #pragma unroll
@@ -634,7 +661,7 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
         if (half == 1) {
           wait_loads();
           __syncthreads();  // layer 3 half 1 in buf1; buf0 free
-          stage_load(A.packed, mn, 0, buf0, wv, l);
+          stage_load(rs, mn, 0, buf0, wv, l, rot);
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
