@@ -549,7 +549,7 @@ __device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0); }
 // one 32x32 output tile of a hidden layer from the fragments of buf (tile
 // t of the half: fragments 16 t .. 16 t + 15).  The A fragments are read
 // PF ahead into a ring of registers, so each MFMA's LDS read has PF - 1
-// matrix operations of cover instead of none.
+// matrix operations of cover (the compiler's own schedule gives one).
 constexpr int PF = 4;
 __device__ __forceinline__ f32x16 tile32(const bf16x8* buf, int t, const bf16x8* in,
                                          f32x16 acc, int l) {
@@ -563,7 +563,9 @@ __device__ __forceinline__ f32x16 tile32(const bf16x8* buf, int t, const bf16x8*
     if (kb + PF < 16) a[kb % PF] = p[(kb + PF) * 64];
   }
   // keep that order in the schedule (left alone, the scheduler sinks each
-  // read next to its MFMA to save registers)
+  // read next to its MFMA to save registers): PF LDS reads (mask 0x100),
+  // then per step one MFMA (0x008), one LDS read and up to five other
+  // vector instructions (0x002)
   __builtin_amdgcn_sched_group_barrier(0x100, PF, 0);
 #pragma unroll
   for (int kb = 0; kb < 16; ++kb) {
@@ -592,7 +594,8 @@ __global__ __launch_bounds__(IW * 64) void k_infer(InferArgs A) {
     }
     if (threadIdx.x < 3) sw[m].b4[threadIdx.x] = (float)w[OFF_B4 + threadIdx.x];
   }
-  // the packed layers 2 and 3 of every member; reads past the end return 0
+  // the packed layers 2 and 3 of every member as a raw buffer (stride 0,
+  // the gfx9 word-3 format bits); reads past the end return 0
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
       (void*)A.packed, (short)0, A.members * 2 * PACKED * (int)sizeof(bf16), 0x00020000);
   const int rot = blockIdx.x & 7;
