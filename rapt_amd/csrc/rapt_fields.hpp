@@ -260,14 +260,54 @@ __device__ __forceinline__ V3 grid_interp(const GridC& G, const double* V, doubl
 }
 
 // ---- user-written fields ------------------------------------------------------
-// One program on the registers (t, x, y, z, ...); every lane runs the same
-// instruction stream, so the instruction words are uniform.  Out of line (one
-// copy per kernel), taking the program by value: a reference into the
-// kernel's argument block would make the caller copy the whole block to
-// scratch to pass its address.  The register file is per-lane scratch (256
-// bytes), indexed by the instruction words.
+// The library functions of the opcode set, out of line: inlined into the
+// interpreter they gave it ~50 registers to save and restore on every call.
+__device__ __noinline__ double expr_fn1(int op, double a) {
+  switch (op) {
+    case 8: return exp(a);
+    case 9: return log(a);
+    case 10: return sin(a);
+    case 11: return cos(a);
+    case 12: return tan(a);
+    case 13: return tanh(a);
+    case 14: return sinh(a);
+    case 15: return cosh(a);
+    case 16: return atan(a);
+    case 17: return asin(a);
+    case 18: return acos(a);
+    default: return log10(a);  // 22
+  }
+}
+__device__ __noinline__ double expr_fn2(int op, double a, double b) {
+  switch (op) {
+    case 6: return pow(a, b);
+    case 20: return atan2(a, b);
+    default: return hypot(a, b);  // 23
+  }
+}
+// a value every lane holds, as a scalar (64-bit)
+template <class P>
+__device__ __forceinline__ P* uniform_ptr(P* p) {
+  const unsigned long long u = (unsigned long long)p;
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+  return (P*)(((unsigned long long)hi << 32) | lo);
+}
+// One program on the registers (t, x, y, z, ...).  The program is a kernel
+// argument, the same for every lane, so it is read through the scalar unit
+// (constant address space): the dispatch is a scalar branch and the register
+// indices are scalar offsets into the lane's scratch registers (256 bytes).
+// Out of line (one copy per kernel), taking the program by value: a
+// reference into the kernel's argument block would make the caller copy the
+// whole block to scratch to pass its address.
 __device__ __noinline__ V3 expr_eval(ExprC X, int lo, int n, double t, double x, double y,
                                      double z) {
+  using CI = const __attribute__((address_space(4))) int32_t;
+  using CD = const __attribute__((address_space(4))) double;
+  CI* code = (CI*)uniform_ptr(X.code);
+  CD* kc = (CD*)uniform_ptr(X.k);
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  n = __builtin_amdgcn_readfirstlane(n);
   double r[EXPR_NREG];
   r[0] = t;
   r[1] = x;
@@ -275,38 +315,40 @@ __device__ __noinline__ V3 expr_eval(ExprC X, int lo, int n, double t, double x,
   r[3] = z;
   r[4] = r[5] = r[6] = 0.0;
   for (int i = lo; i < lo + n; ++i) {
-    const int32_t op = X.code[4 * i], d = X.code[4 * i + 1], ia = X.code[4 * i + 2],
-                  ib = X.code[4 * i + 3];
+    const int32_t op = __builtin_amdgcn_readfirstlane(code[4 * i]),
+                  d = __builtin_amdgcn_readfirstlane(code[4 * i + 1]),
+                  ia = __builtin_amdgcn_readfirstlane(code[4 * i + 2]),
+                  ib = __builtin_amdgcn_readfirstlane(code[4 * i + 3]);
     const double a = r[ia & (EXPR_NREG - 1)], b = r[ib & (EXPR_NREG - 1)];
     double v;
     switch (op) {
-      case 0: v = X.k[ia]; break;
+      case 0: v = kc[ia]; break;
       case 1: v = a + b; break;
       case 2: v = a - b; break;
       case 3: v = a * b; break;
       case 4: v = a / b; break;
       case 5: v = -a; break;
-      case 6: v = pow(a, b); break;
       case 7: v = sqrt(a); break;
-      case 8: v = exp(a); break;
-      case 9: v = log(a); break;
-      case 10: v = sin(a); break;
-      case 11: v = cos(a); break;
-      case 12: v = tan(a); break;
-      case 13: v = tanh(a); break;
-      case 14: v = sinh(a); break;
-      case 15: v = cosh(a); break;
-      case 16: v = atan(a); break;
-      case 17: v = asin(a); break;
-      case 18: v = acos(a); break;
       case 19: v = fabs(a); break;
-      case 20: v = atan2(a, b); break;
       case 21: v = a; break;
-      case 22: v = log10(a); break;
-      case 23: v = hypot(a, b); break;
       case 24: v = (a > b || a != a) ? a : b; break;  // numpy maximum: a NaN propagates,
       case 25: v = (a < b || a != a) ? a : b; break;  // ties give b (numpy minimum alike)
       case 26: v = a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : (a == 0.0 ? 0.0 : a)); break;  // sign
+      case 6:
+      case 20:
+      case 23: v = expr_fn2(op, a, b); break;
+      case 8:
+      case 9:
+      case 10:
+      case 11:
+      case 12:
+      case 13:
+      case 14:
+      case 15:
+      case 16:
+      case 17:
+      case 18:
+      case 22: v = expr_fn1(op, a); break;
       default: v = __builtin_nan(""); break;
     }
     r[d & (EXPR_NREG - 1)] = v;
