@@ -181,8 +181,9 @@ int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt,
  * overrides B and/or E in Python (rapt/fields.py:5-74, the contract of the
  * reference's "Creating new fields" example), whose methods the reference's
  * integrator calls per right-hand side.  The Python front end traces those
- * methods into register programs (rapt_amd/fieldexpr.py); the device
- * interprets them in FP64.  code: (nb + ne) x 4 int32 (opcode, destination,
+ * methods into register programs (rapt_amd/fieldexpr.py); the device runs
+ * them in FP64, compiled into the tracer kernels or interpreted
+ * (rapt_field_expr_mode).  code: (nb + ne) x 4 int32 (opcode, destination,
  * operand a, operand b) over 32 registers, 0-3 = t, x, y, z; each program
  * leaves its vector in registers 4-6; B's program is the first nb
  * instructions, E's the next ne (ne = 0: E = 0).  Opcodes: 0 register[d] =
@@ -200,6 +201,28 @@ int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt,
  * lane per particle whatever lanes_per_particle asks (results are the same). */
 int rapt_field_expr_set(rapt_ctx *ctx, int32_t nb, int32_t ne, const int32_t *code,
                         int32_t nconst, const double *consts);
+
+/* How the tracer kernels evaluate the context's user-written field.  No
+ * counterpart in the reference, whose integrator calls the Python methods
+ * (rapt/fields.py:5-74); both modes give the same trajectories bit for bit.
+ * RAPT_EXPR_COMPILED (the default): the first Particle, GuidingCenter or
+ * Adaptive run of a program compiles the one-lane tracer kernels for it at
+ * run time (hipRTC, a few seconds; code cached for the process by program,
+ * constants excluded) and loads them on the device, so the field is inline
+ * code; rapt_trace_device_noalloc uses them only once an earlier run has
+ * loaded them (loading allocates), and interprets otherwise.
+ * RAPT_EXPR_INTERPRETED: the built-in kernels interpret the program.
+ * Field evaluation and diagnostics always interpret. */
+enum rapt_expr_mode { RAPT_EXPR_INTERPRETED = 0, RAPT_EXPR_COMPILED = 1 };
+int rapt_field_expr_mode(rapt_ctx *ctx, int32_t mode);
+
+/* Compiles the tracer kernels of a program (as rapt_field_expr_set takes it)
+ * for arch ("gfx950" when null or empty) and caches the code for the process,
+ * without a device or a context; the message (the compiler's log on failure)
+ * goes to log (log_size bytes, NUL-terminated).  No counterpart in the
+ * reference: a check, or a warm-up ahead of the first run. */
+int rapt_field_expr_compile(int32_t nb, int32_t ne, const int32_t *code, const char *arch,
+                            char *log, int64_t log_size);
 
 /* Batched field evaluation on the device (the _Field methods,
  * rapt/fields.py:76-280).  what: 0 B, 1 E, 2 unitb, 3 magB, 4 gradB,

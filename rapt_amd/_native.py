@@ -19,6 +19,7 @@ MODE_PARTICLE, MODE_GC, MODE_ADAPTIVE = 0, 1, 2
 EOM = {"TaoChanBrizardEOM": 0, "BrizardChanEOM": 1, "NorthropTellerEOM": 2}
 STEP_DOP853, STEP_DOPRI5, STEP_RKF45 = 0, 1, 2
 PREC_FP64, PREC_FP32_FIELD = 0, 1
+EXPR_INTERPRETED, EXPR_COMPILED = 0, 1
 NSLOT = 16
 S_T, S_Y0, S_MASS, S_CHARGE, S_MU, S_V, S_DT, S_H, S_TEND, S_GM, S_TCUR = \
     0, 1, 7, 8, 9, 10, 11, 12, 13, 14, 15
@@ -118,6 +119,9 @@ def lib():
     L.rapt_grid_set.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp, dp, dp,
                                 dp, dp]
     L.rapt_field_expr_set.argtypes = [vp, C.c_int32, C.c_int32, i32p, C.c_int32, dp]
+    L.rapt_field_expr_mode.argtypes = [vp, C.c_int32]
+    L.rapt_field_expr_compile.argtypes = [C.c_int32, C.c_int32, i32p, C.c_char_p, C.c_char_p,
+                                          C.c_int64]
     L.rapt_eye.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
     L.rapt_particle_gc.argtypes = [vp, C.c_int64, dp, dp, dp, C.c_int32, C.c_double, C.c_int32,
                                    dp, i32p]
@@ -138,7 +142,8 @@ def lib():
                "rapt_batch_create", "rapt_batch_destroy", "rapt_batch_upload",
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
                "rapt_batch_reset", "rapt_batch_set_cost", "rapt_eye", "rapt_particle_gc",
-               "rapt_grid_set", "rapt_field_expr_set", "rapt_trace_device_noalloc"):
+               "rapt_grid_set", "rapt_field_expr_set", "rapt_trace_device_noalloc",
+               "rapt_field_expr_mode", "rapt_field_expr_compile"):
         getattr(L, nm).restype = C.c_int
     _lib = L
     return L
@@ -210,6 +215,13 @@ class Context:
             f.p[i] = v
         self.check(lib().rapt_field_set(self.h, C.byref(f)), "rapt_field_set")
         self._field_key = key
+
+    def set_expr_mode(self, compiled):
+        """How tracer runs evaluate a user-written field: compiled at run time
+        (True, the default) or interpreted (False); the trajectories are the
+        same bit for bit."""
+        self.check(lib().rapt_field_expr_mode(self.h, EXPR_COMPILED if compiled else
+                                              EXPR_INTERPRETED), "rapt_field_expr_mode")
 
     def field_eval(self, desc, what, tpos):
         self.set_field(desc)
@@ -322,6 +334,20 @@ class Context:
             _ip64(acc), float(rtol), float(atol), int(nsteps)), "rapt_bounce_center_equatorial")
         off = np.r_[0, np.cumsum(nrow)]
         return [rows[off[i]:off[i + 1]] for i in range(n)], tp, st, acc
+
+
+def expr_compile(prog, arch="gfx950"):
+    """Compiles the tracer kernels of a traced user-field program for arch
+    (no device needed; the code is cached for the process).  Returns the
+    library's message; raises RuntimeError with the compiler's log on
+    failure."""
+    code = np.ascontiguousarray(prog.code, dtype=np.int32)
+    buf = C.create_string_buffer(8192)
+    rc = lib().rapt_field_expr_compile(prog.nb, prog.ne, _ip32(code), arch.encode(), buf,
+                                       len(buf))
+    if rc != 0:
+        raise RuntimeError("rapt_field_expr_compile failed (%d): %s" % (rc, buf.value.decode()))
+    return buf.value.decode()
 
 
 def context(device=0):

@@ -10,7 +10,9 @@
 // b = B/|B| (for curl b) from it, where the reference evaluates the field
 // twice at each of the six points.
 #pragma once
+#ifndef __HIPCC_RTC__  // (run-time compiled kernels get the runtime from the compiler)
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 // No FMA contraction anywhere after this header (the library is also built
@@ -260,6 +262,17 @@ __device__ __forceinline__ V3 grid_interp(const GridC& G, const double* V, doubl
 }
 
 // ---- user-written fields ------------------------------------------------------
+#ifdef RAPT_JIT_FIELD
+// Compiled at run time (the tracer kernels of one traced program): the
+// source that includes this header defines jit_B and jit_E after it, as
+// straight-line code of the program's instructions, each the same operation
+// on the same values as expr_eval's case for its opcode; k is the program's
+// constant table in device memory.
+constexpr bool EXPR_COMPILED = true;
+__device__ __forceinline__ V3 jit_B(const double* k, double t, double x, double y, double z);
+__device__ __forceinline__ V3 jit_E(const double* k, double t, double x, double y, double z);
+#else
+constexpr bool EXPR_COMPILED = false;
 // The library functions of the opcode set, out of line: inlined into the
 // interpreter they gave it ~50 registers to save and restore on every call.
 __device__ __noinline__ double expr_fn1(int op, double a) {
@@ -355,6 +368,7 @@ __device__ __noinline__ V3 expr_eval(ExprC X, int lo, int n, double t, double x,
   }
   return V3{r[4], r[5], r[6]};
 }
+#endif  // RAPT_JIT_FIELD
 
 // ---- B(t, r) ---------------------------------------------------------------
 // T = double is the tracer's arithmetic; T = float serves the FP32-field
@@ -422,7 +436,11 @@ __device__ __forceinline__ V3T<T> fieldB(const FieldC& F, T t, T x, T y, T z) {
     return grid_interp(F.grid, F.grid.B, t, x, y, z);
   } else if constexpr (M == F_EXPR) {
     static_assert(sizeof(T) == sizeof(double), "user-written fields are FP64");
+#ifdef RAPT_JIT_FIELD
+    return jit_B(F.expr.k, t, x, y, z);
+#else
     return expr_eval(F.expr, 0, F.expr.nb, t, x, y, z);
+#endif
   } else {
     return V{T(0.0), T(0.0), T(0.0)};
   }
@@ -438,7 +456,11 @@ __device__ __forceinline__ V3T<T> fieldE(const FieldC& F, T t, T x, T y, T z) {
   } else if constexpr (M == F_EXPR) {
     static_assert(sizeof(T) == sizeof(double), "user-written fields are FP64");
     if (F.expr.ne == 0) return V3T<T>{T(0.0), T(0.0), T(0.0)};
+#ifdef RAPT_JIT_FIELD
+    return jit_E(F.expr.k, t, x, y, z);
+#else
     return expr_eval(F.expr, F.expr.nb, F.expr.ne, t, x, y, z);
+#endif
   } else if constexpr (M == F_UNIFORM_CROSSED_EB) {
     return V3T<T>{T(0.0), T(F.p[1]), T(0.0)};
   } else if constexpr (M == F_CURRENT_SHEET) {

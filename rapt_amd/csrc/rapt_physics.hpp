@@ -2,7 +2,7 @@
 // conversions on the device, shared by the tracer kernels.
 #pragma once
 #include "rapt_fields.hpp"
-#include "rapt_internal.hpp"
+#include "rapt_kargs.hpp"
 
 namespace rapt {
 

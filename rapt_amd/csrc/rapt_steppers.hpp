@@ -11,7 +11,9 @@
 // Each *_attempt performs one full step attempt (all stages) so that every
 // lane of a wavefront evaluates its right-hand side at the same program point.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 
 namespace rapt {
 

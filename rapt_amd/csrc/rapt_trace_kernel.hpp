@@ -23,12 +23,14 @@
 // and scipy.integrate.ode's call semantics (one integrate() per output row,
 // step size carried between calls, at most nsteps+1 steps per call).
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
-#include "../../include/rapt_hip.h"
+#include "rapt_hip.h"
 #include "rapt_fields.hpp"
-#include "rapt_internal.hpp"
+#include "rapt_kargs.hpp"
 #include "rapt_physics.hpp"
 #include "rapt_steppers.hpp"
 
@@ -83,10 +85,11 @@ struct GcRHS {
 // GC +21% FP64, +31% FP32-field on 1,048,576 particles); one wave for
 // gridded fields, where two spill 180 bytes per lane in the Particle kernel
 // and ran 15% slower (65,536 protons in a 257 x 257 x 193 dipole grid), and
-// for user-written fields, whose interpreter calls carry a register file.
+// for interpreted user-written fields, whose interpreter calls carry a
+// register file (compiled ones are inline code like the analytic models).
 template <int M>
 constexpr int trace_waves() {
-  return M == F_GRID || M == F_EXPR ? 1 : 2;
+  return M == F_GRID || (M == F_EXPR && !EXPR_COMPILED) ? 1 : 2;
 }
 template <int M, int KIND, int P, bool SP = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(trace_waves<M>())))

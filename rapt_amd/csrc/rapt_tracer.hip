@@ -197,6 +197,11 @@ static void launch_trace(rapt_ctx* c, const KArgs& A, int block, int64_t lanes) 
 
 template <int K>
 static void launch_model(rapt_ctx* c, const KArgs& A, int block, int64_t lanes) {
+  // a compiled user-written field, once its kernel is loaded (otherwise the
+  // built-in kernel interprets the program: the same results)
+  if (c->field.model == RAPT_FIELD_EXPR && c->expr_mode == RAPT_EXPR_COMPILED &&
+      rapt_jit_launch(c, K, K == KIND_GC && trace_use_sp<F_EXPR>(A), A, block, lanes))
+    return;
   switch (c->field.model) {
     case 0: launch_trace<0, K>(c, A, block, lanes); break;
     case 1: launch_trace<1, K>(c, A, block, lanes); break;
@@ -331,6 +336,10 @@ int rapt_field_expr_set(rapt_ctx* c, int32_t nb, int32_t ne, const int32_t* code
   c->expr_mem = mem;
   c->expr = ExprC{(const int32_t*)((char*)mem + kb), (const double*)mem, nb, ne};
   c->expr_set = true;
+  c->expr_code.assign(code, code + 4 * (size_t)(nb + ne));
+  c->expr_nb = nb;
+  c->expr_ne = ne;
+  for (hipFunction_t& f : c->jit_fn) f = nullptr;  // the compiled kernels of another program
   if (c->field_set && c->field.model == RAPT_FIELD_EXPR) c->field.expr = c->expr;
   return RAPT_OK;
 }
@@ -777,6 +786,19 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   return RAPT_OK;
 }
 
+// Compiles (once per program) and loads the run-time tracer kernels of a
+// compiled user-written field before a launch that uses them; not called by
+// the allocation-free entry point (loading a module allocates).
+static int prepare_jit(rapt_ctx* c, const rapt_trace_desc* d, const KArgs& A, bool grouped) {
+  if (c->field.model != RAPT_FIELD_EXPR || c->expr_mode != RAPT_EXPR_COMPILED || grouped ||
+      d->precision != RAPT_PREC_FP64)
+    return RAPT_OK;
+  const int kind = d->mode == RAPT_MODE_PARTICLE ? KIND_PARTICLE
+                                                 : (d->mode == RAPT_MODE_GC ? KIND_GC
+                                                                            : KIND_ADAPTIVE);
+  return rapt_jit_prepare(c, kind, kind == KIND_GC && trace_use_sp<F_EXPR>(A));
+}
+
 // The tracer kernel of a prepared run on the context's stream.
 static void launch_tracer(rapt_batch* b, const rapt_trace_desc* d, const KArgs& A, int block,
                           int64_t lanes, bool grouped) {
@@ -823,7 +845,8 @@ int rapt_batch_run(rapt_batch* b, const rapt_trace_desc* d, rapt_counters* cnt) 
   int block = 256;
   int64_t lanes = 0;
   bool grouped = false;
-  const int rc = prepare_run(b, d, A, block, lanes, grouped);
+  int rc = prepare_run(b, d, A, block, lanes, grouped);
+  if (!rc) rc = prepare_jit(c, d, A, grouped);
   if (rc) return rc;
   HIP_OK(c, hipMemsetAsync(b->d_work, 0, sizeof(unsigned long long) * 8, c->stream));
   HIP_OK(c, hipEventRecord(c->ev0, c->stream));
@@ -1019,6 +1042,7 @@ int rapt_batch_epochs_begin(rapt_batch* b, const rapt_trace_desc* d, int32_t row
   bool grouped = false;
   int rc = prepare_run(b, d, A, block, lanes, grouped);
   b->width = w0;
+  if (!rc) rc = prepare_jit(c, d, A, grouped);  // before the capture: loading allocates
   if (rc) {
     rapt_batch_epochs_end(b);
     return rc;

@@ -253,3 +253,18 @@ def test_selection_ops_follow_numpy_on_special_values():
         want = np.array([np.maximum(x, y), np.minimum(x, y), np.sign(z)])
         np.testing.assert_array_equal(got, want)
         np.testing.assert_array_equal(np.signbit(got), np.signbit(want))
+
+
+def test_user_field_program_compiles_for_gfx950(user_fields):
+    """The run-time compiled tracer kernels of a traced program (the
+    reference's charged-dipole example: B and E programs) build from the
+    headers embedded in the library, on the CPU (no device involved); a bad
+    opcode is refused with the library's message."""
+    from rapt_amd import _native as N
+    prog = user_fields["ChargedDipole"]().descriptor()["expr"]
+    msg = N.expr_compile(prog)
+    assert "4 kernels" in msg, msg
+    code = prog.code.copy()
+    code[0, 0] = 99
+    with pytest.raises(RuntimeError, match="rapt_field_expr_compile"):
+        N.expr_compile(FE.Program(code, prog.consts, prog.nb, prog.ne))
