@@ -45,6 +45,7 @@ FLOP_RHS_TCB_DOUBLEDIPOLE = 484      # one Tao-Chan-Brizard RHS, 7 field evaluat
 FLOP_DOPRI5_STEP_N4 = 282            # stage combinations, error norm, control
 FLOP_RHS_TCB_REFERENCE = 795         # same RHS as the reference writes it (13 field evals)
 FLOP_RHS_LORENTZ_DIPOLE = 34         # one Lorentz RHS in EarthDipole (static gamma m)
+FLOP_RHS_LORENTZ_NOFIELD = 15        # the same without the field: v = p/(gamma m), p x B, q/(gamma m)
 FLOP_DOP853_STEP_N6 = 988            # DOP853 attempt without its RHS: stage sums 666,
                                      # stage times 23, solution + two error norms 282,
                                      # norm combination 7, step-size control 10
@@ -377,6 +378,83 @@ def run_grid(args):
     return res
 
 
+def run_user_field(args):
+    """The reference's plugin contract at speed: EarthDipole + (0, 0, -10 nT)
+    written as a user _Field subclass (B overridden in Python, calling the
+    parent's B), config 1's protons at n = 16,384, Particle.advance(1).  The
+    program compiled into the tracer kernel at run time against the same
+    program interpreted (bit-identical rows) and the built-in model of the
+    same field (same formula, different rounding)."""
+    import rapt_amd as rapt
+    from rapt_amd import _native as N, batch as BT, workloads as W
+
+    class DipoleSouthward(rapt.fields.EarthDipole):
+        def __init__(self, Bu=(0.0, 0.0, -10e-9)):
+            rapt.fields.EarthDipole.__init__(self)
+            self._Bu = np.array(Bu, dtype=float)
+
+        def B(self, tpos):
+            return rapt.fields.EarthDipole.B(self, tpos) + self._Bu
+
+    n = args.n or 16384
+    wl = W.dipole_protons_1mev(n=n)
+    g = 1 / np.sqrt(1 - np.sum(wl["vel"] ** 2, axis=1) / rapt.c ** 2)
+    last7 = np.zeros((n, 7))
+    last7[:, 1:4] = wl["pos"]
+    last7[:, 4:7] = (rapt.m_pr * g)[:, None] * wl["vel"]
+    user, builtin = DipoleSouthward(), rapt.fields.DipolePlusUniform((0.0, 0.0, -10e-9))
+    prog = user.descriptor()["expr"]
+    S = BT.particle_state(last7, rapt.m_pr, rapt.e)
+    dt = BT.particle_dt(builtin, last7, rapt.m_pr, rapt.e, 20)
+    S[N.S_DT] = dt
+    cap = BT._rows_needed(1.0, dt)
+    desc = N.make_desc(N.MODE_PARTICLE, dict(rapt.params, cyclotronresolution=20), 1.0)
+    ctx = N.context(0)
+    t0 = time.perf_counter()
+    N.expr_compile(prog, "gfx950")  # the kernels the first compiled run loads
+    compile_s = time.perf_counter() - t0
+    runs = {}
+    try:
+        for name, f, compiled in (("compiled", user, True), ("interpreted", user, False),
+                                  ("builtin", builtin, True)):
+            ctx.set_expr_mode(compiled)
+            runs[name] = timed_run(f, N.MODE_PARTICLE, S, np.zeros(n, np.int32), cap, desc,
+                                   args.steps, args.warmup, 7, rows=name != "builtin")
+    finally:
+        ctx.set_expr_mode(True)
+    tot, el, out = runs["compiled"]
+    oi = runs["interpreted"][2]
+    same = bool(np.array_equal(out["state"], oi["state"]) and
+                np.array_equal(out["nrows"], oi["nrows"]) and
+                np.array_equal(out["rows"], oi["rows"]))
+    pa, pb = out["state"][1:4], runs["builtin"][2]["state"][1:4]
+    rel = np.linalg.norm(pa - pb, axis=0) / np.linalg.norm(pb, axis=0)
+    value = tot["naccpt"] / el
+    ops = int(prog.nb)
+    ksec = tot["kernel_ms"] / args.steps / 1e3
+    flop = (tot["nfcn"] * (FLOP_RHS_LORENTZ_NOFIELD + ops) + tot["nstep"] * FLOP_DOP853_STEP_N6) \
+        / args.steps
+    res = base_line("FP64 particle-steps/s in a user-written field (1 GPU)", value,
+                    "particle-steps/s", args, el, "f64",
+                    {"workload": "EarthDipole + (0,0,-10 nT) written as a _Field subclass "
+                                 "(B in Python, %d traced operations); %d protons 1 MeV, "
+                                 "r U[4,7] Re, pitch U[10,90] deg; DOP853 default tolerances, "
+                                 "advance(1), dt=T_c/20" % (ops, n), "particles": n})
+    res.update({
+        "kernel_ms_avg": {k: r[0]["kernel_ms"] / args.steps for k, r in runs.items()},
+        "particle_steps_per_s": {k: r[0]["naccpt"] / r[1] for k, r in runs.items()},
+        "compile_s_four_kernels": compile_s,
+        "compiled_equals_interpreted": same,
+        "final_position_rel_diff_vs_builtin": _pct(rel),
+        "particles_done": int(np.sum(out["status"] == 1)),
+        "roofline": {"bound": "valu_fp64", "achieved": flop / ksec / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": flop / ksec / PEAK_FP64, "traffic": None,
+                     "flop_per_launch": flop,
+                     "note": "%d lanes: %.2f waves per SIMD" % (n, n / 64 / 1024)}})
+    return res
+
+
 def run_adaptive_sheet(args):
     """262,144 protons, B = (20 nT tanh(z/0.5 Re), 0, 1 nT), E = 0.3 mV/m y,
     Adaptive.advance(120) with GCtimestep 0.5 s."""
@@ -668,7 +746,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="gc_doubledipole",
                     choices=("gc_doubledipole", "particle_dipole", "adaptive_sheet",
-                             "mixed_precision", "surrogate", "grid"))
+                             "mixed_precision", "surrogate", "grid", "user_field"))
     ap.add_argument("--epochs", type=int, default=100, help="surrogate training epochs")
     ap.add_argument("--dump", default="", help="surrogate: write held-out results to this .npz")
     ap.add_argument("--infer-only", action="store_true",
@@ -724,7 +802,8 @@ def main():
 
     runners = {"particle_dipole": run_particle_dipole, "adaptive_sheet": run_adaptive_sheet,
                "grid": run_grid, "mixed_precision": run_mixed_precision,
-               "surrogate": run_surrogate, "gc_doubledipole": run_gc_doubledipole}
+               "surrogate": run_surrogate, "gc_doubledipole": run_gc_doubledipole,
+               "user_field": run_user_field}
     res = runners[args.workload](args)
     if args.workload == "gc_doubledipole" and not args.headline_only and not args.top \
             and not args.skip_top:
@@ -746,6 +825,10 @@ def other_configs(args):
               "switches_per_step", "status_counts", "config")),
             ("mixed_precision", run_mixed_precision,
              ("value", "gc_runs", "errors_per_species", "config")),
+            ("user_field", run_user_field,
+             ("value", "kernel_ms_avg", "particle_steps_per_s", "compile_s_four_kernels",
+              "compiled_equals_interpreted", "final_position_rel_diff_vs_builtin", "roofline",
+              "config")),
             ("surrogate", run_surrogate,
              ("value", "unit", "speedup_vs_traced", "traced_queries_per_s", "inference_kernel_ms",
               "fallback_queries", "fallback_ms", "heldout_surrogate_rel_err",

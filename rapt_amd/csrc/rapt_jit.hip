@@ -220,7 +220,11 @@ int rapt_jit_prepare(rapt_ctx* c, int kind, bool sp) {
     return set_err(c, RAPT_E_STATE, "user-written field: no program uploaded");
   hipDeviceProp_t prop;
   HIP_OK(c, hipGetDeviceProperties(&prop, c->device));
-  const std::string arch = prop.gcnArchName;
+  // the processor without its feature flags (gfx950, not gfx950:sramecc+:...),
+  // as the library itself is built: the code runs whatever the flags, and a
+  // device-free rapt_field_expr_compile for "gfx950" fills the same cache entry
+  std::string arch = prop.gcnArchName;
+  arch = arch.substr(0, arch.find(':'));
   std::string src;
   if (!make_source(c->expr_nb, c->expr_ne, c->expr_code.data(), src))
     return set_err(c, RAPT_E_ARG, "user-written field: unknown opcode");
