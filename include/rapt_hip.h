@@ -197,8 +197,9 @@ int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt,
  * the library copies both into device memory owned by the context.  Then
  * rapt_field_set with model RAPT_FIELD_EXPR (params unused) selects it.
  * Bounce-path entry points (rapt_halfbounce, rapt_eye, rapt_bounce_*) do not
- * take it; FP32-field precision does not either; GuidingCenter runs use one
- * lane per particle whatever lanes_per_particle asks (results are the same). */
+ * take it; FP32-field precision does not either; interpreted GuidingCenter
+ * runs use one lane per particle whatever lanes_per_particle asks (results
+ * are the same), compiled ones every layout. */
 int rapt_field_expr_set(rapt_ctx *ctx, int32_t nb, int32_t ne, const int32_t *code,
                         int32_t nconst, const double *consts);
 

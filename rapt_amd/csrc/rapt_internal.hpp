@@ -11,6 +11,28 @@
 #include "rapt_fields.hpp"
 #include "rapt_kargs.hpp"
 
+// kernels of a user-written field compiled at run time (rapt_jit.hip)
+#define RAPT_JIT_SLOTS 12
+
+// Grid of a lane-group GuidingCenter launch: single layouts (L = 2, 4, 8)
+// `groups` resident groups of L lanes; mixed layouts (lanes_per_particle 3,
+// 5) `groups` octets, `quads` quads and `pairs` pairs, whose blocks are
+// [0, oct_blocks) octets, [oct_blocks, wide_blocks) quads, the rest pairs.
+struct GcnGrid {
+  unsigned blocks;
+  int32_t oct_blocks, wide_blocks;
+};
+static inline GcnGrid gcn_grid(int lanes_per_particle, int block, int64_t groups, int64_t quads,
+                               int64_t pairs) {
+  if (lanes_per_particle == 3 || lanes_per_particle == 5) {
+    const int64_t ob = (8 * groups + block - 1) / block, qb = (4 * quads + block - 1) / block,
+                  pb = (2 * pairs + block - 1) / block;
+    return GcnGrid{(unsigned)(ob + qb + pb), (int32_t)ob, (int32_t)(ob + qb)};
+  }
+  const int L = lanes_per_particle == 2 ? 2 : (lanes_per_particle == 8 ? 8 : 4);
+  return GcnGrid{(unsigned)((groups * L + block - 1) / block), 0, 0};
+}
+
 struct rapt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -30,7 +52,8 @@ struct rapt_ctx {
   std::vector<int32_t> expr_code;
   int32_t expr_nb = 0, expr_ne = 0;
   int32_t expr_mode = RAPT_EXPR_COMPILED;
-  hipFunction_t jit_fn[4] = {};  // Particle, GC, GC static TCB, Adaptive
+  hipFunction_t jit_fn[RAPT_JIT_SLOTS] = {};  // one lane: Particle, GC, GC static TCB,
+                                              // Adaptive; then the GC lane-group kernels
   // device address of this device's allocation-free scratch (the work-queue
   // and step counters of rapt_trace_device_noalloc), from hipGetSymbolAddress
   unsigned long long* noalloc_scratch = nullptr;
@@ -52,8 +75,11 @@ static inline int set_err(rapt_ctx* c, int code, const char* fmt, ...) {
 
 // run-time compiled tracer kernels of a user-written field (rapt_jit.hip)
 int rapt_jit_prepare(rapt_ctx* c, int kind, bool sp);
+int rapt_jit_prepare_gcn(rapt_ctx* c, int lanes_per_particle, bool sp);
 bool rapt_jit_launch(rapt_ctx* c, int kind, bool sp, const rapt::KArgs& A, int block,
                      int64_t lanes);
+bool rapt_jit_launch_gcn(rapt_ctx* c, int lanes_per_particle, bool sp, const rapt::KArgs& A,
+                         int block, int64_t groups, int64_t quads, int64_t pairs);
 
 #define HIP_OK(ctx, call)                                                              \
   do {                                                                                 \

@@ -7,10 +7,12 @@
 // becomes C++, one statement per instruction -- the same operation on the
 // same values as the interpreter's case for its opcode, the constants read
 // from the program's table in device memory -- and the one-lane tracer
-// kernels k_trace<F_EXPR, kind, FP64> are compiled for it with hipRTC from
+// kernels k_trace<F_EXPR, kind, FP64> and the lane-group GuidingCenter
+// kernels k_trace_gcn / k_trace_gcmix are compiled for it with hipRTC from
 // the library's own headers (embedded when the library is built), so the
 // field is inline code like an analytic model's and the kernels run at two
-// waves per SIMD.  The trajectories are the interpreter's, bit for bit.
+// waves per SIMD.  The trajectories are the interpreter's one-lane kernel's,
+// bit for bit, in every layout.
 //
 // Each kernel is compiled when a run first needs it (a few seconds).  Code
 // objects are cached for the process by program text and kernel (the
@@ -36,19 +38,38 @@ using namespace rapt;
 
 namespace {
 
-// the instantiations the one-lane launch code uses for F_EXPR (FP64):
-// Particle, GuidingCenter (generic, and the static Tao-Chan-Brizard one),
-// Adaptive; index = jit_slot(kind, sp)
+// the instantiations the launch code uses for F_EXPR (FP64): one lane per
+// particle for Particle, GuidingCenter (generic, and the static
+// Tao-Chan-Brizard one) and Adaptive (slots 0-3, rapt_trace_kernel.hpp);
+// lane pairs, quads and octets (4-9) and the mixed layout (10-11) for
+// GuidingCenter (rapt_gcn_kernel.hpp)
 static_assert(F_EXPR == 10 && KIND_PARTICLE == 0 && KIND_GC == 1 && KIND_ADAPTIVE == 2 &&
                   RAPT_PREC_FP64 == 0,
               "kernel names below spell these values");
-constexpr int kSlots = 4;
+constexpr int kSlots = RAPT_JIT_SLOTS;
+constexpr int kFirstGroupSlot = 4;
 const char* const kKernelNames[kSlots] = {
     "rapt::k_trace<10, 0, 0, false>", "rapt::k_trace<10, 1, 0, false>",
-    "rapt::k_trace<10, 1, 0, true>", "rapt::k_trace<10, 2, 0, false>"};
+    "rapt::k_trace<10, 1, 0, true>",  "rapt::k_trace<10, 2, 0, false>",
+    "rapt::k_trace_gcn<10, 2, false>", "rapt::k_trace_gcn<10, 2, true>",
+    "rapt::k_trace_gcn<10, 4, false>", "rapt::k_trace_gcn<10, 4, true>",
+    "rapt::k_trace_gcn<10, 8, false>", "rapt::k_trace_gcn<10, 8, true>",
+    "rapt::k_trace_gcmix<10, false>",  "rapt::k_trace_gcmix<10, true>"};
 
 int jit_slot(int kind, bool sp) {
   return kind == KIND_PARTICLE ? 0 : (kind == KIND_GC ? (sp ? 2 : 1) : 3);
+}
+// lanes_per_particle as rapt_trace_desc has it (2, 8; 3 and 5 mixed; 0 and
+// 4 quads)
+int jit_group_slot(int lanes_per_particle, bool sp) {
+  const int s = sp ? 1 : 0;
+  switch (lanes_per_particle) {
+    case 2: return 4 + s;
+    case 8: return 8 + s;
+    case 3:
+    case 5: return 10 + s;
+    default: return 6 + s;
+  }
 }
 
 std::string reg(int i) { return "r" + std::to_string(i); }
@@ -105,9 +126,9 @@ bool emit_body(const int32_t* code, int lo, int n, std::string& s) {
   return true;
 }
 
-// The whole translation unit: the field functions between the field header
-// and the tracer kernel header that calls them.
-bool make_source(int32_t nb, int32_t ne, const int32_t* code, std::string& src) {
+// The whole translation unit of a kernel slot: the field functions between
+// the field header and the kernel header that calls them.
+bool make_source(int32_t nb, int32_t ne, const int32_t* code, int slot, std::string& src) {
   src = "#define RAPT_JIT_FIELD 1\n#include \"rapt_fields.hpp\"\nnamespace rapt {\n";
   const char* fn[2] = {"jit_B", "jit_E"};
   for (int p = 0; p < 2; ++p) {
@@ -123,7 +144,8 @@ bool make_source(int32_t nb, int32_t ne, const int32_t* code, std::string& src) 
     }
     src += "}\n";
   }
-  src += "}  // namespace rapt\n#include \"rapt_trace_kernel.hpp\"\n";
+  src += slot < kFirstGroupSlot ? "}  // namespace rapt\n#include \"rapt_trace_kernel.hpp\"\n"
+                                : "}  // namespace rapt\n#include \"rapt_gcn_kernel.hpp\"\n";
   return true;
 }
 
@@ -213,8 +235,7 @@ int compile_locked(const std::string& arch, int slot, const std::string& src,
 // for a kind on its device.  Called where a launch may allocate (the module
 // load does); the allocation-free entry point uses a kernel only once this
 // has run.
-int rapt_jit_prepare(rapt_ctx* c, int kind, bool sp) {
-  const int slot = jit_slot(kind, sp);
+static int prepare_slot(rapt_ctx* c, int slot) {
   if (c->jit_fn[slot]) return RAPT_OK;
   if (!c->expr_set || c->expr_code.empty())
     return set_err(c, RAPT_E_STATE, "user-written field: no program uploaded");
@@ -226,7 +247,7 @@ int rapt_jit_prepare(rapt_ctx* c, int kind, bool sp) {
   std::string arch = prop.gcnArchName;
   arch = arch.substr(0, arch.find(':'));
   std::string src;
-  if (!make_source(c->expr_nb, c->expr_ne, c->expr_code.data(), src))
+  if (!make_source(c->expr_nb, c->expr_ne, c->expr_code.data(), slot, src))
     return set_err(c, RAPT_E_ARG, "user-written field: unknown opcode");
   std::lock_guard<std::mutex> lock(g_mu);
   const JitCode* jc = nullptr;
@@ -251,6 +272,12 @@ int rapt_jit_prepare(rapt_ctx* c, int kind, bool sp) {
   return RAPT_OK;
 }
 
+int rapt_jit_prepare(rapt_ctx* c, int kind, bool sp) { return prepare_slot(c, jit_slot(kind, sp)); }
+
+int rapt_jit_prepare_gcn(rapt_ctx* c, int lanes_per_particle, bool sp) {
+  return prepare_slot(c, jit_group_slot(lanes_per_particle, sp));
+}
+
 // One-lane tracer launch of a compiled kernel, with the same grid as the
 // built-in kernels' launch; false (nothing launched) when rapt_jit_prepare
 // has not loaded it.  Errors surface through hipGetLastError like those of
@@ -262,6 +289,26 @@ bool rapt_jit_launch(rapt_ctx* c, int kind, bool sp, const KArgs& A, int block, 
   void* args[] = {const_cast<KArgs*>(&A)};
   (void)hipModuleLaunchKernel(f, (unsigned)g, 1, 1, (unsigned)block, 1, 1, 0, c->stream, args,
                               nullptr);
+  return true;
+}
+
+// The lane-group launch (rapt_launch_gcn's grids) of a compiled kernel; false
+// when it is not loaded.
+bool rapt_jit_launch_gcn(rapt_ctx* c, int lanes_per_particle, bool sp, const KArgs& A,
+                         int block, int64_t groups, int64_t quads, int64_t pairs) {
+  hipFunction_t f = c->jit_fn[jit_group_slot(lanes_per_particle, sp)];
+  if (!f) return false;
+  const GcnGrid g = gcn_grid(lanes_per_particle, block, groups, quads, pairs);
+  if (lanes_per_particle == 3 || lanes_per_particle == 5) {
+    int32_t ob = g.oct_blocks, wb = g.wide_blocks;
+    void* args[] = {const_cast<KArgs*>(&A), &ob, &wb};
+    (void)hipModuleLaunchKernel(f, g.blocks, 1, 1, (unsigned)block, 1, 1, 0, c->stream, args,
+                                nullptr);
+  } else {
+    void* args[] = {const_cast<KArgs*>(&A)};
+    (void)hipModuleLaunchKernel(f, g.blocks, 1, 1, (unsigned)block, 1, 1, 0, c->stream, args,
+                                nullptr);
+  }
   return true;
 }
 
@@ -289,11 +336,11 @@ int rapt_field_expr_compile(int32_t nb, int32_t ne, const int32_t* code, const c
       return RAPT_E_ARG;
     }
   }
-  std::string src;
-  if (!make_source(nb, ne, code, src)) return RAPT_E_ARG;
   std::lock_guard<std::mutex> lock(g_mu);
   size_t bytes = 0;
   for (int slot = 0; slot < kSlots; ++slot) {
+    std::string src;
+    if (!make_source(nb, ne, code, slot, src)) return RAPT_E_ARG;
     const JitCode* jc = nullptr;
     std::string msg;
     const int rc = compile_locked(arch && *arch ? arch : "gfx950", slot, src, &jc, msg);

@@ -748,13 +748,14 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
   int64_t lanes = d->lanes > 0 ? d->lanes : b->n;
   if (lanes > b->n) lanes = b->n;
   if (lanes < 1) lanes = 1;
-  // the grouped kernels (two, four or eight lanes, mixed) are FP64 only; a
-  // user-written field runs one lane per particle (the grouped kernels keep
-  // too many registers live across the interpreter's calls: 1-1.2 KB of
-  // spills per lane)
+  // the grouped kernels (two, four or eight lanes, mixed) are FP64 only; an
+  // interpreted user-written field runs one lane per particle (the grouped
+  // kernels keep too many registers live across the interpreter's calls:
+  // 1-1.2 KB of spills per lane), a compiled one takes every layout
   const bool grouped = d->mode == RAPT_MODE_GC && d->stepper == RAPT_STEP_DOPRI5 &&
                        d->lanes_per_particle != 1 && d->precision == RAPT_PREC_FP64 &&
-                       c->field.model != RAPT_FIELD_EXPR;
+                       (c->field.model != RAPT_FIELD_EXPR ||
+                        c->expr_mode == RAPT_EXPR_COMPILED);
   // Default block: 256 lanes, except for launches of fewer than four waves
   // per CU, which take 64 so that the waves spread over every CU (a CU's
   // waves share its scalar unit and instruction fetch): 1,024 Particle-mode
@@ -790,9 +791,10 @@ static int prepare_run(rapt_batch* b, const rapt_trace_desc* d, KArgs& A, int& b
 // compiled user-written field before a launch that uses them; not called by
 // the allocation-free entry point (loading a module allocates).
 static int prepare_jit(rapt_ctx* c, const rapt_trace_desc* d, const KArgs& A, bool grouped) {
-  if (c->field.model != RAPT_FIELD_EXPR || c->expr_mode != RAPT_EXPR_COMPILED || grouped ||
+  if (c->field.model != RAPT_FIELD_EXPR || c->expr_mode != RAPT_EXPR_COMPILED ||
       d->precision != RAPT_PREC_FP64)
     return RAPT_OK;
+  if (grouped) return rapt_jit_prepare_gcn(c, d->lanes_per_particle, trace_use_sp<F_EXPR>(A));
   const int kind = d->mode == RAPT_MODE_PARTICLE ? KIND_PARTICLE
                                                  : (d->mode == RAPT_MODE_GC ? KIND_GC
                                                                             : KIND_ADAPTIVE);
@@ -814,13 +816,20 @@ static void launch_tracer(rapt_batch* b, const rapt_trace_desc* d, const KArgs& 
         const int64_t p = (d->lanes - 8 * octets - 4 * quads) / 2;
         pairs = p < 1 ? 1 : (p < pairs ? p : pairs);
       }
-      rapt_launch_gcn(c->stream, c->field.model, d->lanes_per_particle, A, block, octets, quads,
-                      pairs);
+      if (c->field.model == RAPT_FIELD_EXPR)  // compiled (prepare_jit loaded it)
+        rapt_jit_launch_gcn(c, d->lanes_per_particle, trace_use_sp<F_EXPR>(A), A, block, octets,
+                            quads, pairs);
+      else
+        rapt_launch_gcn(c->stream, c->field.model, d->lanes_per_particle, A, block, octets,
+                        quads, pairs);
     } else {
       const int L = d->lanes_per_particle == 2 ? 2 : (d->lanes_per_particle == 8 ? 8 : 4);
       int64_t groups = d->lanes > 0 ? (d->lanes + L - 1) / L : b->n;
       if (groups > b->n) groups = b->n;
-      rapt_launch_gcn(c->stream, c->field.model, L, A, block, groups, 0, 0);
+      if (c->field.model == RAPT_FIELD_EXPR)
+        rapt_jit_launch_gcn(c, L, trace_use_sp<F_EXPR>(A), A, block, groups, 0, 0);
+      else
+        rapt_launch_gcn(c->stream, c->field.model, L, A, block, groups, 0, 0);
     }
   } else if (d->precision == RAPT_PREC_FP32_FIELD) {
     const int kind = d->mode == RAPT_MODE_PARTICLE ? KIND_PARTICLE
@@ -929,6 +938,9 @@ int rapt_trace_device_noalloc(rapt_ctx* c, const rapt_trace_desc* d, int64_t n,
   view.d_work = c->noalloc_scratch;
   rapt_trace_desc dd = *d;
   if (!order) dd.work_order = 1;  // no order given: input order
+  // a user-written field: one lane per particle (compiled if an earlier run
+  // loaded that kernel, else interpreted; loading a module allocates)
+  if (c->field.model == RAPT_FIELD_EXPR) dd.lanes_per_particle = 1;
   KArgs A;
   int block = 256;
   int64_t lanes = 0;

@@ -343,3 +343,35 @@ def test_compiled_field_speed(user_fields):
     ti, tc, tb = (r.counters["kernel_ms"] for r in (a, b, builtin))
     print("kernel ms: interpreted %.2f compiled %.2f built-in %.2f" % (ti, tc, tb))
     assert tc * 4 < ti, (ti, tc)
+
+
+@pytest.mark.parametrize("cls", ["TwinDipole", "BreathingSheet"])
+def test_compiled_gc_lane_groups_bit_identical(golden, user_fields, cls):
+    """A compiled user field takes the lane-group GuidingCenter kernels
+    (pairs, quads, octets, the two mixed layouts; the static Tao-Chan-Brizard
+    instantiation for TwinDipole, the generic one for BreathingSheet): rows,
+    statuses and step counts equal the interpreted one-lane run's bit for
+    bit, with refill (fewer resident lanes than particles) included."""
+    g = golden("userfields")
+    name = "twin_gc" if cls == "TwinDipole" else "breathing_gc"
+    ref = g[name + "/traj"]
+    v, pa, m, q, mu, gdt, delta = g[name + "/in"][3:]
+    n = 40
+    rows0 = np.repeat(ref[:1], n, axis=0)
+    rows0[:, 1:4] *= np.linspace(0.9, 1.1, n)[:, None]
+    f = user_fields[cls]()
+    run = lambda **kw: batch.advance_gcs(f, rows0, m, q, [mu] * n, [v] * n, [gdt] * n, delta,
+                                         params=P(GCtimestep=gdt), **kw)
+    ctx = N.context()
+    try:
+        ctx.set_expr_mode(False)
+        base = run(lanes_per_particle=1)
+        ctx.set_expr_mode(True)
+        assert np.all(base.status == N.ST_DONE)
+        for kw in (dict(lanes_per_particle=2), dict(lanes_per_particle=4),
+                   dict(lanes_per_particle=8), dict(lanes_per_particle=4, lanes=64),
+                   dict(lanes_per_particle=3, mixed_quads=8),
+                   dict(lanes_per_particle=5, mixed_octets=4, mixed_quads=8)):
+            _assert_same(base, run(**kw))
+    finally:
+        ctx.set_expr_mode(True)

@@ -263,7 +263,7 @@ def test_user_field_program_compiles_for_gfx950(user_fields):
     from rapt_amd import _native as N
     prog = user_fields["ChargedDipole"]().descriptor()["expr"]
     msg = N.expr_compile(prog)
-    assert "4 kernels" in msg, msg
+    assert "12 kernels" in msg, msg
     code = prog.code.copy()
     code[0, 0] = 99
     with pytest.raises(RuntimeError, match="rapt_field_expr_compile"):
