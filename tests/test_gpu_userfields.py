@@ -375,3 +375,59 @@ def test_compiled_gc_lane_groups_bit_identical(golden, user_fields, cls):
             _assert_same(base, run(**kw))
     finally:
         ctx.set_expr_mode(True)
+
+
+def test_compiled_user_field_output_epochs(golden, user_fields):
+    """The compiled lane-group kernel captured in the hipGraph of output
+    epochs (a module launch inside the capture): every epoch's rows equal
+    the single compiled launch's, and those equal the interpreted run's."""
+    g = golden("userfields")
+    ref = g["twin_gc/traj"]
+    v, pa, m, q, mu, gdt, delta = g["twin_gc/in"][3:]
+    n = 24
+    rows0 = np.repeat(ref[:1], n, axis=0)
+    rows0[:, 1:4] *= np.linspace(0.9, 1.1, n)[:, None]
+    f = user_fields["TwinDipole"]()
+    prm = P(GCtimestep=gdt)
+    a, one = _both_modes(lambda: batch.advance_gcs(f, rows0, m, q, [mu] * n, [v] * n,
+                                                   [gdt] * n, delta, params=prm))
+    _assert_same(a, one)
+    S = batch.gc_state(rows0, m, q, [mu] * n, [v] * n, [gdt] * n)
+    desc = N.make_desc(N.MODE_GC, prm, delta, lanes_per_particle=4)
+    ctx = N.context()
+    ctx.set_field(f.descriptor())
+    b = N.Batch(ctx, n)
+    got = [[] for _ in range(n)]
+    epochs = 0
+    try:
+        b.upload(S, np.zeros(n, np.int32))
+        for rows, nrows, status, left, cnt in b.epochs(desc, 5):
+            epochs += 1
+            for i in range(n):
+                k0 = sum(len(r) for r in got[i])
+                seg = rows[i, :nrows[i]]
+                np.testing.assert_array_equal(seg, one.rows[i][k0:k0 + nrows[i]])
+                got[i].append(seg)
+    finally:
+        b.close()
+    assert epochs >= 3
+    for i in range(n):
+        np.testing.assert_array_equal(np.vstack(got[i]), one.rows[i])
+
+
+def test_compiled_user_field_noalloc(user_fields):
+    """The allocation-free entry point with a user-written field: it loads no
+    module, so it runs the compiled one-lane kernel an earlier run loaded (or
+    interprets): the rows equal the batch path's either way."""
+    from test_zz_no_alloc import _noalloc
+    last = _protons(32, 29, 3.0, 8.0, 0, 2)
+    f = user_fields["ChargedDipole"](Q=1e-7)
+    a = batch.advance_particles(f, last, K.m_pr, K.e, 1e-4, params=P())
+    S = batch.particle_state(last, K.m_pr, K.e)
+    dt = batch.particle_dt(f, last, K.m_pr, K.e, P()["cyclotronresolution"])
+    cap = batch._rows_needed(1e-4, dt)
+    desc = N.make_desc(N.MODE_PARTICLE, P(), 1e-4)
+    out = _noalloc(f, N.MODE_PARTICLE, S, np.zeros(32, np.int32), cap, desc)
+    np.testing.assert_array_equal(out["status"], a.status)
+    for x, y in zip(out["rows"], a.rows):
+        np.testing.assert_array_equal(x, y)
