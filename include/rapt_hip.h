@@ -196,8 +196,8 @@ int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt,
  * At most RAPT_EXPR_MAXCODE instructions and RAPT_EXPR_MAXCONST constants;
  * the library copies both into device memory owned by the context.  Then
  * rapt_field_set with model RAPT_FIELD_EXPR (params unused) selects it.
- * Bounce-path entry points (rapt_halfbounce, rapt_eye, rapt_bounce_*) do not
- * take it; FP32-field precision does not either; interpreted GuidingCenter
+ * The field-line entry points (rapt_halfbounce, rapt_eye, rapt_bounce_*)
+ * interpret it; FP32-field precision does not take it; interpreted GuidingCenter
  * runs use one lane per particle whatever lanes_per_particle asks (results
  * are the same), compiled ones every layout. */
 int rapt_field_expr_set(rapt_ctx *ctx, int32_t nb, int32_t ne, const int32_t *code,
@@ -236,7 +236,8 @@ int rapt_field_eval(rapt_ctx *ctx, int what, int64_t n, const double *tpos, doub
  * rapt/GuidingCenter.py:593-606): field-line trace, mirror points, integral.
  * tpos: n x 4 host array; Bm: mirror fields; sb: out (m); status: 1 ok, < 0
  * on failure.  Needs a dipole-type field (EarthDipole, DoubleDipole,
- * VarEarthDipole, EarthDipole + uniform). */
+ * VarEarthDipole, EarthDipole + uniform), a gridded or a user-written field
+ * (interpreted). */
 int rapt_halfbounce(rapt_ctx *ctx, int64_t n, const double *tpos, const double *Bm,
                     double fieldlineresolution, double *sb, int32_t *status);
 
@@ -260,7 +261,8 @@ int rapt_eye(rapt_ctx *ctx, int64_t n, const double *tpos, const double *Bm,
  * out[3i..3i+2] = (bounce period [s], bounce-averaged drift angular velocity
  * about z [rad/s], +b mirror-point latitude [deg]).  Re and eV are the
  * caller's constants (rapt/__init__.py:4-10).  Static dipole-type fields
- * (models 1, 2, 5, 8).  status[i] as rapt_halfbounce. */
+ * (models 1, 2, 5, 8), gridded and user-written ones.  status[i] as
+ * rapt_halfbounce. */
 int rapt_bounce_labels(rapt_ctx *ctx, int64_t n, const double *query, double mass,
                        double charge, double Re, double eV, double eyegradientstep,
                        double fieldlineresolution, double *out, int32_t *status);
@@ -274,7 +276,7 @@ int rapt_bounce_labels(rapt_ctx *ctx, int64_t n, const double *query, double mas
  * delta, dt))); rows: [sum nrow][4] (t, x, y, z) with the reference's row
  * times (the arange value, one step behind the state); status 1 ok, -2
  * nsteps, -3 step too small, < -19 field-line failure.  Static dipole-type
- * fields (models 1, 2, 8). */
+ * fields (models 1, 2, 8) and static user-written ones (interpreted). */
 int rapt_bounce_center_advance(rapt_ctx *ctx, int64_t n, double *tpos, const double *Bm,
                                const double *coef, const double *dt, const double *tcur,
                                const int64_t *nrow, double *rows, int32_t *status,

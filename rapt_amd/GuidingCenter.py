@@ -18,7 +18,7 @@ from .constants import c
 
 _EOMS = ("TaoChanBrizardEOM", "BrizardChanEOM", "NorthropTellerEOM")
 # field models whose half-bounce path the device computes (rapt_halfbounce)
-_BOUNCE_MODELS = (1, 2, 5, 8, 9)
+_BOUNCE_MODELS = (1, 2, 5, 8, 9, 10)
 
 
 class GuidingCenter:

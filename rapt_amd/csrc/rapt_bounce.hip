@@ -763,9 +763,10 @@ static int run_bounce(rapt_ctx* c, int64_t n, const double* in, int in_w, const 
         case 5: lab ? launch_lab<5>(c, A, *lab) : launch_hb<5>(c, A); break;
         case 8: lab ? launch_lab<8>(c, A, *lab) : launch_hb<8>(c, A); break;
         case 9: lab ? launch_lab<9>(c, A, *lab) : launch_hb<9>(c, A); break;
+        case 10: lab ? launch_lab<10>(c, A, *lab) : launch_hb<10>(c, A); break;
         default:
-          rc = set_err(c, RAPT_E_ARG, "field-line integrals need a dipole-type field (model %d)",
-                       c->field.model);
+          rc = set_err(c, RAPT_E_ARG, "field-line integrals need a dipole-type, gridded or "
+                       "user-written field (model %d)", c->field.model);
       }
       if (rc == RAPT_OK) {
         if (hipGetLastError() != hipSuccess ||
@@ -925,9 +926,10 @@ extern "C" int rapt_bounce_center_advance(rapt_ctx* c, int64_t n, double* tpos, 
         case 1: launch_bc<1>(c, A, C); break;
         case 2: launch_bc<2>(c, A, C); break;
         case 8: launch_bc<8>(c, A, C); break;
+        case 10: launch_bc<10>(c, A, C); break;
         default:
-          rc = set_err(c, RAPT_E_ARG, "BounceCenter needs a static dipole-type field (model %d)",
-                       c->field.model);
+          rc = set_err(c, RAPT_E_ARG, "BounceCenter needs a static dipole-type or user-written "
+                       "field (model %d)", c->field.model);
       }
       if (rc == RAPT_OK) {
         if (hipGetLastError() != hipSuccess ||

@@ -18,6 +18,7 @@ import rapt_amd as rapt
 from rapt_amd import _native as N
 from rapt_amd import batch
 from rapt_amd import constants as K
+from rapt_amd import fields as F
 
 pytestmark = pytest.mark.gpu
 
@@ -180,15 +181,61 @@ def test_program_switch_between_launches(user_fields):
 
 def test_user_field_rejections(user_fields):
     """What the device does not take for a user field fails loudly: FP32
-    field precision and the field-line integrals."""
+    field precision, and BounceCenter in a field that is not static."""
     f = user_fields["TwinDipole"]()
     last = np.zeros((1, 7))
     last[0, 1:4] = [5 * K.Re, 0, 0]
     last[0, 5] = K.m_pr * 1e7
     with pytest.raises(RuntimeError, match="FP64 only"):
         batch.advance_particles(f, last, K.m_pr, K.e, 1.0, params=P(), precision="fp32_field")
-    with pytest.raises(RuntimeError, match="dipole-type"):
-        N.context().halfbounce(f.descriptor(), last[:, :4], np.array([1e-6]), 1e-3)
+    with pytest.raises(Exception, match="static"):
+        bc = rapt.BounceCenter([5 * K.Re, 0, 0], 1e7, 0.0, 30.0, K.m_pr, K.e,
+                               user_fields["BreathingSheet"]())
+        bc.advance(10.0)
+
+
+def test_user_field_bounce_paths(user_fields):
+    """The field-line kernels with a user-written field (interpreted): the
+    reference's two-dipole field written by the user against the built-in
+    DoubleDipole (the same formula; only rounding differs, so the traces
+    agree far inside the reference's own quadrature tolerance): the half-
+    bounce path, the second invariant (GuidingCenter.geteye) and
+    BounceCenter.advance, and the device output step of a batched
+    GuidingCenter advance, which now takes user fields."""
+    fu, fb = user_fields["TwinDipole"](), F.DoubleDipole()
+    ctx = N.context()
+    L = np.array([4.0, 5.0, 6.0, 7.0])
+    tp = np.zeros((4, 4))
+    tp[:, 1] = L * K.Re
+    tp[:, 3] = [0.0, 0.1, -0.2, 0.05]
+    tp[:, 3] *= K.Re
+    Bm = np.array([3e-6, 1.5e-6, 8e-7, 4e-7])
+    res = rapt.params["fieldlineresolution"]
+    for fn in (ctx.halfbounce, ctx.eye):
+        a, sa = fn(fu.descriptor(), tp, Bm, res)
+        b, sb = fn(fb.descriptor(), tp, Bm, res)
+        assert np.all(sa == 1) and np.all(sb == 1), (sa, sb)
+        np.testing.assert_allclose(a, b, rtol=1e-6)
+    # BounceCenter.advance, three output rows
+    saved = rapt.params["BCtimestep"]
+    rapt.params["BCtimestep"] = 5.0
+    try:
+        traj = []
+        for f in (fu, fb):
+            bc = rapt.BounceCenter([5 * K.Re, 0, 0], 1e7, 0.0, 40.0, K.m_pr, K.e, f)
+            bc.advance(150.0)  # output step 5 bounce periods, ~58 s
+            traj.append(bc.trajectory)
+    finally:
+        rapt.params["BCtimestep"] = saved
+    assert traj[0].shape == traj[1].shape and len(traj[0]) >= 3
+    np.testing.assert_allclose(traj[0][:, 1:4], traj[1][:, 1:4], rtol=1e-6, atol=1e-6 * K.Re)
+    # batched GuidingCenter advance: device output steps in the user field
+    gs = [rapt.GuidingCenter([l * K.Re, 0, 0], 1e7, 45.0, None, 0, K.m_pr, K.e, f)
+          for f in (fu, fb) for l in (4.0, 6.0)]
+    dt_u = rapt.GuidingCenter._batch_dt(gs[:2], True)
+    dt_b = rapt.GuidingCenter._batch_dt(gs[2:], True)
+    np.testing.assert_allclose(dt_u, dt_b, rtol=1e-6)
+    np.testing.assert_allclose(dt_u, [g._dt() for g in gs[:2]], rtol=1e-4)
 
 
 def test_device_selection_ops():
