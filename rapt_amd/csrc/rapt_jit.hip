@@ -93,7 +93,7 @@ bool emit_body(const int32_t* code, int lo, int n, std::string& s) {
       case 3: e = a + " * " + b; break;
       case 4: e = a + " / " + b; break;
       case 5: e = "-" + a; break;
-      case 6: e = "pow(" + a + ", " + b + ")"; break;
+      case 6: e = "expr_pow(" + a + ", " + b + ")"; break;
       case 7: f1 = "sqrt"; break;
       case 8: f1 = "exp"; break;
       case 9: f1 = "log"; break;
