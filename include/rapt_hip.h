@@ -197,7 +197,8 @@ int rapt_grid_set(rapt_ctx *ctx, int32_t nx, int32_t ny, int32_t nz, int32_t nt,
  * the library copies both into device memory owned by the context.  Then
  * rapt_field_set with model RAPT_FIELD_EXPR (params unused) selects it.
  * The field-line entry points (rapt_halfbounce, rapt_eye, rapt_bounce_*)
- * interpret it; FP32-field precision does not take it; interpreted GuidingCenter
+ * take it (compiled or interpreted as rapt_field_expr_mode says); FP32-field
+ * precision does not; interpreted GuidingCenter
  * runs use one lane per particle whatever lanes_per_particle asks (results
  * are the same), compiled ones every layout. */
 int rapt_field_expr_set(rapt_ctx *ctx, int32_t nb, int32_t ne, const int32_t *code,
@@ -213,17 +214,23 @@ int rapt_field_expr_set(rapt_ctx *ctx, int32_t nb, int32_t ne, const int32_t *co
  * code; rapt_trace_device_noalloc uses them only once an earlier run has
  * loaded them (loading allocates), and interprets otherwise.
  * RAPT_EXPR_INTERPRETED: the built-in kernels interpret the program.
- * Field evaluation and diagnostics always interpret. */
+ * The field-line entry points (rapt_halfbounce, rapt_eye, rapt_bounce_*)
+ * compile their kernels in the same way; field evaluation and diagnostics
+ * always interpret. */
 enum rapt_expr_mode { RAPT_EXPR_INTERPRETED = 0, RAPT_EXPR_COMPILED = 1 };
 int rapt_field_expr_mode(rapt_ctx *ctx, int32_t mode);
 
-/* Compiles the tracer kernels of a program (as rapt_field_expr_set takes it)
- * for arch ("gfx950" when null or empty) and caches the code for the process,
- * without a device or a context; the message (the compiler's log on failure)
- * goes to log (log_size bytes, NUL-terminated).  No counterpart in the
- * reference: a check, or a warm-up ahead of the first run. */
+/* Compiles kernels of a program (as rapt_field_expr_set takes it) for arch
+ * ("gfx950" when null or empty) and caches the code for the process, without
+ * a device or a context.  kernels: bit mask, 0 = all: bits 0-3 the one-lane
+ * Particle, GuidingCenter, static Tao-Chan-Brizard GuidingCenter and
+ * Adaptive kernels; 4-9 GuidingCenter lane pairs, quads and octets (generic,
+ * static TCB); 10-11 the mixed layout; 12-15 half-bounce path / eye, labels,
+ * BounceCenter, equatorial BounceCenter.  The message (the compiler's log on
+ * failure) goes to log (log_size bytes, NUL-terminated).  No counterpart in
+ * the reference: a check, or a warm-up ahead of the first run. */
 int rapt_field_expr_compile(int32_t nb, int32_t ne, const int32_t *code, const char *arch,
-                            char *log, int64_t log_size);
+                            int32_t kernels, char *log, int64_t log_size);
 
 /* Batched field evaluation on the device (the _Field methods,
  * rapt/fields.py:76-280).  what: 0 B, 1 E, 2 unitb, 3 magB, 4 gradB,

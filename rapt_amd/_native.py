@@ -120,8 +120,8 @@ def lib():
                                 dp, dp]
     L.rapt_field_expr_set.argtypes = [vp, C.c_int32, C.c_int32, i32p, C.c_int32, dp]
     L.rapt_field_expr_mode.argtypes = [vp, C.c_int32]
-    L.rapt_field_expr_compile.argtypes = [C.c_int32, C.c_int32, i32p, C.c_char_p, C.c_char_p,
-                                          C.c_int64]
+    L.rapt_field_expr_compile.argtypes = [C.c_int32, C.c_int32, i32p, C.c_char_p, C.c_int32,
+                                          C.c_char_p, C.c_int64]
     L.rapt_eye.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
     L.rapt_particle_gc.argtypes = [vp, C.c_int64, dp, dp, dp, C.c_int32, C.c_double, C.c_int32,
                                    dp, i32p]
@@ -336,15 +336,16 @@ class Context:
         return [rows[off[i]:off[i + 1]] for i in range(n)], tp, st, acc
 
 
-def expr_compile(prog, arch="gfx950"):
-    """Compiles the tracer kernels of a traced user-field program for arch
-    (no device needed; the code is cached for the process).  Returns the
+def expr_compile(prog, arch="gfx950", kernels=0):
+    """Compiles kernels of a traced user-field program for arch (no device
+    needed; the code is cached for the process).  kernels: the library's bit
+    mask (0: all sixteen; 1: the one-lane Particle kernel; ...).  Returns the
     library's message; raises RuntimeError with the compiler's log on
     failure."""
     code = np.ascontiguousarray(prog.code, dtype=np.int32)
     buf = C.create_string_buffer(8192)
-    rc = lib().rapt_field_expr_compile(prog.nb, prog.ne, _ip32(code), arch.encode(), buf,
-                                       len(buf))
+    rc = lib().rapt_field_expr_compile(prog.nb, prog.ne, _ip32(code), arch.encode(),
+                                       int(kernels), buf, len(buf))
     if rc != 0:
         raise RuntimeError("rapt_field_expr_compile failed (%d): %s" % (rc, buf.value.decode()))
     return buf.value.decode()

@@ -8,7 +8,8 @@
 // same values as the interpreter's case for its opcode, the constants read
 // from the program's table in device memory -- and the one-lane tracer
 // kernels k_trace<F_EXPR, kind, FP64> and the lane-group GuidingCenter
-// kernels k_trace_gcn / k_trace_gcmix are compiled for it with hipRTC from
+// kernels k_trace_gcn / k_trace_gcmix, and the field-line kernels (half-
+// bounce path, eye, surrogate labels, BounceCenter) are compiled for it with hipRTC from
 // the library's own headers (embedded when the library is built), so the
 // field is inline code like an analytic model's and the kernels run at two
 // waves per SIMD.  The trajectories are the interpreter's one-lane kernel's,
@@ -42,19 +43,26 @@ namespace {
 // particle for Particle, GuidingCenter (generic, and the static
 // Tao-Chan-Brizard one) and Adaptive (slots 0-3, rapt_trace_kernel.hpp);
 // lane pairs, quads and octets (4-9) and the mixed layout (10-11) for
-// GuidingCenter (rapt_gcn_kernel.hpp)
+// GuidingCenter (rapt_gcn_kernel.hpp); the field-line kernels (12-15,
+// rapt_bounce_kernel.hpp)
 static_assert(F_EXPR == 10 && KIND_PARTICLE == 0 && KIND_GC == 1 && KIND_ADAPTIVE == 2 &&
                   RAPT_PREC_FP64 == 0,
               "kernel names below spell these values");
 constexpr int kSlots = RAPT_JIT_SLOTS;
 constexpr int kFirstGroupSlot = 4;
+constexpr int kFirstLineSlot = RAPT_JIT_HALFBOUNCE;
+static_assert(RAPT_JIT_HALFBOUNCE == 12 && RAPT_JIT_LABELS == 13 && RAPT_JIT_BC == 14 &&
+                  RAPT_JIT_BC_EQ == 15 && kSlots == 16,
+              "slot numbers below");
 const char* const kKernelNames[kSlots] = {
     "rapt::k_trace<10, 0, 0, false>", "rapt::k_trace<10, 1, 0, false>",
     "rapt::k_trace<10, 1, 0, true>",  "rapt::k_trace<10, 2, 0, false>",
     "rapt::k_trace_gcn<10, 2, false>", "rapt::k_trace_gcn<10, 2, true>",
     "rapt::k_trace_gcn<10, 4, false>", "rapt::k_trace_gcn<10, 4, true>",
     "rapt::k_trace_gcn<10, 8, false>", "rapt::k_trace_gcn<10, 8, true>",
-    "rapt::k_trace_gcmix<10, false>",  "rapt::k_trace_gcmix<10, true>"};
+    "rapt::k_trace_gcmix<10, false>",  "rapt::k_trace_gcmix<10, true>",
+    "rapt::k_halfbounce<10>",          "rapt::k_bounce_labels<10>",
+    "rapt::k_bounce_center<10>",       "rapt::k_bounce_center_eq<10>"};
 
 int jit_slot(int kind, bool sp) {
   return kind == KIND_PARTICLE ? 0 : (kind == KIND_GC ? (sp ? 2 : 1) : 3);
@@ -144,8 +152,10 @@ bool make_source(int32_t nb, int32_t ne, const int32_t* code, int slot, std::str
     }
     src += "}\n";
   }
-  src += slot < kFirstGroupSlot ? "}  // namespace rapt\n#include \"rapt_trace_kernel.hpp\"\n"
-                                : "}  // namespace rapt\n#include \"rapt_gcn_kernel.hpp\"\n";
+  src += "}  // namespace rapt\n";
+  src += slot < kFirstGroupSlot ? "#include \"rapt_trace_kernel.hpp\"\n"
+         : slot < kFirstLineSlot ? "#include \"rapt_gcn_kernel.hpp\"\n"
+                                 : "#include \"rapt_bounce_kernel.hpp\"\n";
   return true;
 }
 
@@ -312,6 +322,17 @@ bool rapt_jit_launch_gcn(rapt_ctx* c, int lanes_per_particle, bool sp, const KAr
   return true;
 }
 
+// A field-line kernel of the context's program (slot RAPT_JIT_HALFBOUNCE ..
+// RAPT_JIT_BC_EQ): compiled and loaded when first needed, then launched with
+// the built-in launch's grid and arguments.
+int rapt_jit_run_line(rapt_ctx* c, int slot, unsigned grid, unsigned block, void** args) {
+  const int rc = prepare_slot(c, slot);
+  if (rc) return rc;
+  HIP_OK(c, hipModuleLaunchKernel(c->jit_fn[slot], grid, 1, 1, block, 1, 1, 0, c->stream, args,
+                                  nullptr));
+  return RAPT_OK;
+}
+
 extern "C" {
 
 int rapt_field_expr_mode(rapt_ctx* c, int32_t mode) {
@@ -324,7 +345,7 @@ int rapt_field_expr_mode(rapt_ctx* c, int32_t mode) {
 }
 
 int rapt_field_expr_compile(int32_t nb, int32_t ne, const int32_t* code, const char* arch,
-                            char* log, int64_t log_size) {
+                            int32_t kernels, char* log, int64_t log_size) {
   put_log("", log, log_size);
   if (!code || nb < 1 || ne < 0 || nb + ne > RAPT_EXPR_MAXCODE) return RAPT_E_ARG;
   for (int32_t i = 0; i < nb + ne; ++i) {
@@ -338,7 +359,10 @@ int rapt_field_expr_compile(int32_t nb, int32_t ne, const int32_t* code, const c
   }
   std::lock_guard<std::mutex> lock(g_mu);
   size_t bytes = 0;
+  int count = 0;
   for (int slot = 0; slot < kSlots; ++slot) {
+    if (kernels != 0 && !((kernels >> slot) & 1)) continue;
+    ++count;
     std::string src;
     if (!make_source(nb, ne, code, slot, src)) return RAPT_E_ARG;
     const JitCode* jc = nullptr;
@@ -350,7 +374,7 @@ int rapt_field_expr_compile(int32_t nb, int32_t ne, const int32_t* code, const c
     }
     bytes += jc->obj.size();
   }
-  put_log(std::to_string(bytes) + " bytes of code in " + std::to_string(kSlots) + " kernels",
+  put_log(std::to_string(bytes) + " bytes of code in " + std::to_string(count) + " kernels",
           log, log_size);
   return RAPT_OK;
 }

@@ -216,6 +216,10 @@ def test_user_field_bounce_paths(user_fields):
         b, sb = fn(fb.descriptor(), tp, Bm, res)
         assert np.all(sa == 1) and np.all(sb == 1), (sa, sb)
         np.testing.assert_allclose(a, b, rtol=1e-6)
+        # the compiled kernels (the default) against the interpreted ones
+        ai, ac = _both_modes(lambda: fn(fu.descriptor(), tp, Bm, res))
+        np.testing.assert_array_equal(ai[0], ac[0])
+        np.testing.assert_array_equal(ai[1], ac[1])
     # BounceCenter.advance, three output rows
     saved = rapt.params["BCtimestep"]
     rapt.params["BCtimestep"] = 5.0
@@ -229,6 +233,18 @@ def test_user_field_bounce_paths(user_fields):
         rapt.params["BCtimestep"] = saved
     assert traj[0].shape == traj[1].shape and len(traj[0]) >= 3
     np.testing.assert_allclose(traj[0][:, 1:4], traj[1][:, 1:4], rtol=1e-6, atol=1e-6 * K.Re)
+
+    def bc_run():
+        rapt.params["BCtimestep"] = 5.0
+        try:
+            bc = rapt.BounceCenter([5 * K.Re, 0, 0], 1e7, 0.0, 40.0, K.m_pr, K.e, fu)
+            bc.advance(150.0)
+        finally:
+            rapt.params["BCtimestep"] = saved
+        return bc.trajectory
+    ti, tc = _both_modes(bc_run)
+    np.testing.assert_array_equal(ti, tc)
+    np.testing.assert_array_equal(tc, traj[0])
     # batched GuidingCenter advance: device output steps in the user field
     gs = [rapt.GuidingCenter([l * K.Re, 0, 0], 1e7, 45.0, None, 0, K.m_pr, K.e, f)
           for f in (fu, fb) for l in (4.0, 6.0)]
