@@ -444,6 +444,7 @@ def run_user_field(args):
     res.update({
         "kernel_ms_avg": {k: r[0]["kernel_ms"] / args.steps for k, r in runs.items()},
         "particle_steps_per_s": {k: r[0]["naccpt"] / r[1] for k, r in runs.items()},
+        "gc_doubledipole": user_field_gc(args),
         "first_compiled_run_s": compile_s,
         "compiled_equals_interpreted": same,
         "final_position_rel_diff_vs_builtin": _pct(rel),
@@ -454,6 +455,49 @@ def run_user_field(args):
                      "flop_per_launch": flop,
                      "note": "%d lanes: %.2f waves per SIMD" % (n, n / 64 / 1024)}})
     return res
+
+
+def user_field_gc(args):
+    """The headline's guiding-centre batch (65,536 electrons, advance(60),
+    lane quads) in the two-dipole field written as a user _Field subclass
+    (the image dipole shifted in place, pow(r2, 5/2)), compiled at run time,
+    against the built-in DoubleDipole in the same layout and work order."""
+    import rapt_amd as rapt
+    from rapt_amd import _native as N, batch as BT
+
+    class TwinDipole(rapt.fields._Field):
+        def __init__(self, B0=3.07e-5, d=20 * rapt.Re, k=1):
+            rapt.fields._Field.__init__(self)
+            self.gradientstepsize = rapt.Re / 1000
+            self._d, self._k, self._c = d, k, -B0 * rapt.Re ** 3
+
+        def B(self, tpos):
+            t, x, y, z = tpos
+            B1 = np.array([3 * x * z, 3 * y * z, (2 * z * z - x * x - y * y)]) / \
+                pow(x * x + y * y + z * z, 5.0 / 2.0)
+            x -= self._d
+            B2 = self._k * np.array([3 * x * z, 3 * y * z, (2 * z * z - x * x - y * y)]) / \
+                pow(x * x + y * y + z * z, 5.0 / 2.0)
+            return self._c * (B1 + B2)
+
+    n = 65536
+    w = build_workload(n)
+    S = BT.gc_state(w["last5"], w["m"], w["q"], w["mu"], w["v"], w["dt"])
+    cap = BT._rows_needed(w["delta"], w["dt"])
+    desc = N.make_desc(N.MODE_GC, w["params"], w["delta"], lanes_per_particle=4)
+    out = {}
+    for name, f in (("compiled", TwinDipole()), ("builtin", w["field"])):
+        tot, el, o = timed_run(f, N.MODE_GC, S, np.zeros(n, np.int32), cap, desc,
+                               min(args.steps, 2), 1, 5)
+        out[name] = {"particle_steps_per_s": tot["naccpt"] / el,
+                     "kernel_ms_avg": tot["kernel_ms"] / min(args.steps, 2),
+                     "state": o["state"][1:4]}
+    a, b = out["compiled"].pop("state"), out["builtin"].pop("state")
+    rel = np.linalg.norm(a - b, axis=0) / np.linalg.norm(b, axis=0)
+    out["final_position_rel_diff_vs_builtin"] = _pct(rel)
+    out["workload"] = ("%d DoubleDipole electrons of the headline batch, advance(60), lane "
+                       "quads, output-step order" % n)
+    return out
 
 
 def run_adaptive_sheet(args):
@@ -827,7 +871,8 @@ def other_configs(args):
             ("mixed_precision", run_mixed_precision,
              ("value", "gc_runs", "errors_per_species", "config")),
             ("user_field", run_user_field,
-             ("value", "kernel_ms_avg", "particle_steps_per_s", "first_compiled_run_s",
+             ("value", "kernel_ms_avg", "particle_steps_per_s", "gc_doubledipole",
+              "first_compiled_run_s",
               "compiled_equals_interpreted", "final_position_rel_diff_vs_builtin", "roofline",
               "config")),
             ("surrogate", run_surrogate,
