@@ -101,3 +101,46 @@ def gradI(tpos, field, Bm, usedipole=False):
             return (I1 - I(tpos)) / d
         return (I1 - I2) / (2 * d)
     return deriv(v1) * v1[1:] + deriv(v2) * v2[1:]
+
+
+# ---- field-line diagnostics (host, as in the reference) -------------------------------
+def criticalpoints(tpos, field, Bm):
+    """(s, B, B'') at each extremum of |B| along the field line through tpos,
+    traced to |B| = Bm (rapt/flutils.py:21-63): the trace points where the
+    sign of dB changes seed a local minimisation of the cubic interpolant of
+    B(s); B'' is its central second difference with step 0.1 ds (the
+    reference's scipy.misc.derivative(n=2), removed from scipy, is that
+    three-point formula)."""
+    from scipy.optimize import minimize
+    fl = Fieldline(tpos, field, Bmax=Bm)
+    fl.trace()
+    s, b = fl.gets(), fl.getB()
+    db = np.diff(b)
+    seeds = [s[i] for i in range(1, len(b) - 1) if db[i - 1] * db[i] <= 0]
+    B = interp1d(s, b, kind="cubic", assume_sorted=True)
+    h = 0.1 * (s[1] - s[0])
+    out = []
+    for s0 in seeds:
+        sm = minimize(B, s0).x[0]
+        out.append((sm, float(B(sm)), float((B(sm + h) - 2 * B(sm) + B(sm - h)) / h ** 2)))
+    return out
+
+
+def fl_error(tpos, field, Bm):
+    """(max, mean) relative distance of the traced field-line points from the
+    dipole line r = r0 cos^2(latitude) through tpos (rapt/flutils.py:318-352;
+    the mean divides by the number of points less one, as there)."""
+    fl = Fieldline(tpos, field, Bmax=Bm)
+    fl.trace()
+    r0 = np.sqrt(np.dot(tpos[1:], tpos[1:]))
+    pts = fl.curve[:, 1:]
+    r = np.sqrt(np.sum(pts * pts, axis=1))
+    err = np.abs(1 - r / (r0 * np.cos(np.arcsin(pts[:, 2] / r)) ** 2))
+    return (float(np.max(err)) if len(err) else 0.0,
+            float(np.sum(err) / (len(fl.curve) - 1)))
+
+
+def eye_error(tpos, field, Bm):
+    """I from the traced field line minus the dipole formula's
+    (rapt/flutils.py:354-376)."""
+    return eye(tpos, field, Bm, usedipole=False) - eye(tpos, field, Bm, usedipole=True)

@@ -193,3 +193,12 @@ def dipole_gradI(x, y, Bm, h):
     r0 = np.sqrt(x ** 2 + y ** 2)
     f = lambda dx: dipoleI(r0 + dx, Bm)
     return (f(h) - f(-h)) / (2 * h) * np.array((x, y, 0)) / r0
+
+
+def drift_error(b):
+    """Relative deviation of a BounceCenter trajectory's distance from the
+    origin from the first row's, row by row -- the error against the circular
+    drift path of a dipole (rapt/utils.py:583-606)."""
+    xyz = np.asarray(b.trajectory)[:, 1:]
+    r0 = np.sqrt(np.dot(xyz[0], xyz[0]))
+    return (np.sqrt(np.sum(xyz * xyz, axis=1)) - r0) / r0
