@@ -494,3 +494,22 @@ def test_compiled_user_field_noalloc(user_fields):
     np.testing.assert_array_equal(out["status"], a.status)
     for x, y in zip(out["rows"], a.rows):
         np.testing.assert_array_equal(x, y)
+
+
+def test_compiled_program_and_constant_switches(user_fields):
+    """Compiled kernels follow the context's program: two charges of the
+    charged dipole (same code, other constants: the same compiled kernel
+    reads the new constant table) and another field in turn, each run equal
+    to its interpreted run."""
+    last = _protons(16, 31, 3.0, 8.0, 0, 2)
+    fields = [user_fields["ChargedDipole"](Q=1e-7), user_fields["ChargedDipole"](Q=3e-7),
+              user_fields["BreathingSheet"](), user_fields["ChargedDipole"](Q=1e-7)]
+    runs = []
+    for f in fields:
+        a, b = _both_modes(lambda: batch.advance_particles(f, last, K.m_pr, K.e, 5e-5,
+                                                           params=P()))
+        _assert_same(a, b)
+        runs.append(b)
+    for x, y in zip(runs[0].rows, runs[3].rows):
+        np.testing.assert_array_equal(x, y)
+    assert not all(np.array_equal(x, y) for x, y in zip(runs[0].rows, runs[1].rows))
