@@ -534,7 +534,63 @@ def run_adaptive_sheet(args):
     res["kernel_ms_avg"] = tot["kernel_ms"] / args.steps
     st = out["status"]
     res["status_counts"] = {str(k): int(np.sum(st == k)) for k in np.unique(st)}
+    if not args.no_cpu:
+        res.update(adaptive_oracle_leg(f, wl, prm, m, min(1024, n)))
     return res
+
+
+def adaptive_oracle_leg(f, wl, prm, m, k):
+    """CPU baseline and full-length parity of the Adaptive batch on its first
+    k particles: the oracle's Adaptive.advance(120) of each, timed on one
+    host core, against the device's rows of the same particles (traced in a
+    launch of their own: a particle's rows do not depend on the batch around
+    it).  Orbits after a switch are chaotic (the reference says so), so the
+    bar is the initial mode, the first switch's row and time, and the rows up
+    to it (1e-7)."""
+    import rapt_amd as rapt
+    from rapt_amd import batch as BT
+    from oracle import oracle as O
+    gpu = BT.advance_adaptive(f, wl["pos"][:k], wl["vel"][:k], 0.0, m, rapt.e, 120.0,
+                              params=prm)
+    fo = O.field_from_desc(f.descriptor())
+    po = O.particle_params(gctimestep=0.5)
+
+    def first_switch(rows):
+        j = np.nonzero(np.diff(rows[:, 0]) != 0)[0]
+        return int(j[0]) + 1 if j.size else None
+
+    t0 = time.perf_counter()
+    ref = [O.adaptive_run(fo, po, wl["pos"][i], wl["vel"][i], 0.0, m, rapt.e, 120.0)
+           for i in range(k)]
+    t_cpu = time.perf_counter() - t0
+    steps = sum(info["naccpt"] for _, info in ref)
+    same_mode = same_first = same_count = 0
+    err = []
+    for i, (ro, info) in enumerate(ref):
+        got = gpu.rows[i]
+        same_mode += int(got[0, 0] == ro[0, 0])
+        kg, ko = first_switch(got), first_switch(ro)
+        same_first += int(kg == ko and (ko is None or
+                                        abs(got[ko, 1] - ro[ko, 1]) <= 1e-12 * abs(ro[ko, 1])))
+        same_count += int(np.sum(np.diff(got[:, 0]) != 0) == info["nswitch"])
+        j = min(len(ro) if ko is None else ko + 1, len(got))
+        d = np.linalg.norm(got[:j, 2:5] - ro[:j, 2:5], axis=1) / \
+            np.linalg.norm(ro[:j, 2:5], axis=1)
+        err.append(float(np.max(d)))
+    err = np.array(err)
+    return {
+        "parity": {"particles": k, "same_initial_mode": same_mode,
+                   "same_first_switch_row_and_time_1e-12": same_first,
+                   "same_switch_count": same_count,
+                   "max_pos_rel_err_to_first_switch": float(err.max()),
+                   "frac_within_1e-07_to_first_switch": float(np.mean(err <= 1e-7)),
+                   "what": "first %d particles of the timed batch, full advance(120), against "
+                           "the oracle" % k},
+        "cpu_baseline": {"value": steps / t_cpu, "unit": "particle-steps/s", "cores": 1,
+                         "kind": "port",
+                         "sample": "first %d particles, full advance(120), C oracle on one "
+                                   "core (%.1f s)" % (k, t_cpu)},
+    }
 
 
 def _pct(a):
@@ -867,7 +923,7 @@ def other_configs(args):
               "parity", "mu_drift", "cpu_baseline", "gpu_vs_cpu", "particles_done", "config")),
             ("adaptive_sheet", run_adaptive_sheet,
              ("value", "ms_per_step", "kernel_ms_avg", "accepted_steps_per_step",
-              "switches_per_step", "status_counts", "config")),
+              "switches_per_step", "status_counts", "parity", "cpu_baseline", "config")),
             ("mixed_precision", run_mixed_precision,
              ("value", "gc_runs", "errors_per_species", "config")),
             ("user_field", run_user_field,
