@@ -238,6 +238,19 @@ int rapt_field_expr_compile(int32_t nb, int32_t ne, const int32_t *code, const c
  * 11 timescale.  tpos: n x 4 host array; out: n x width(what) host array. */
 int rapt_field_eval(rapt_ctx *ctx, int what, int64_t n, const double *tpos, double *out);
 
+/* The tracers' right-hand sides at n points: the Lorentz equations of
+ * Particle.advance's eom (rapt/Particle.py:284-298) for mode
+ * RAPT_MODE_PARTICLE, ty [n][7] = (t, x, y, z, px, py, pz), k [n][3] = (mass,
+ * charge, gamma m; a static field keeps the given gamma m, as the reference
+ * freezes it, a time-dependent one recomputes it), out [n][6]; the
+ * guiding-centre equations GuidingCenter._TaoChanBrizardEOM /
+ * _BrizardChanEOM / _NorthropTellerEOM (rapt/GuidingCenter.py:329-395, eom
+ * as enum rapt_eom) for RAPT_MODE_GC, ty [n][5] = (t, x, y, z, ppar), k [n][4]
+ * = (mass, charge, mu, v), out [n][4].  enforce_equatorial zeroes the z
+ * components as params["enforce equatorial"] does. */
+int rapt_rhs_eval(rapt_ctx *ctx, int32_t mode, int32_t eom, int32_t enforce_equatorial,
+                  int64_t n, const double *ty, const double *k, double *out);
+
 /* Half-bounce path length S_b for n guiding centres (flutils.halfbouncepath,
  * rapt/flutils.py:254-316, as GuidingCenter.bounceperiod uses it,
  * rapt/GuidingCenter.py:593-606): field-line trace, mirror points, integral.

@@ -274,6 +274,28 @@ class GuidingCenter:
             v = c * np.sqrt(1 - 1 / g ** 2)
         return ru.cyclotron_period2(t, r, v, self.field, self.mass, self.charge)
 
+    # ---- equations of motion (rapt/GuidingCenter.py:329-395), on the device ------------
+    def _eom(self, eom, t, Y):
+        from . import _native as N
+        from . import params as P
+        ty = np.r_[t, np.asarray(Y, dtype=float)[:4]][None, :]
+        return N.context().rhs_eval(self.field.descriptor(), N.MODE_GC, ty,
+                                    [self.mass, self.charge, self.mu, self.v], eom=eom,
+                                    enforce_equatorial=P["enforce equatorial"])[0]
+
+    def _TaoChanBrizardEOM(self, t, Y):
+        """d(x, y, z, ppar)/dt in the Tao-Chan-Brizard equations (E fields and
+        time-dependent fields allowed)."""
+        return self._eom("TaoChanBrizardEOM", t, Y)
+
+    def _BrizardChanEOM(self, t, Y):
+        """d(x, y, z, ppar)/dt in the Brizard-Chan equations (static B)."""
+        return self._eom("BrizardChanEOM", t, Y)
+
+    def _NorthropTellerEOM(self, t, Y):
+        """d(x, y, z, ppar)/dt in the Northrop-Teller equations."""
+        return self._eom("NorthropTellerEOM", t, Y)
+
     def bounceperiod(self):
         """Bounce period at the current position (rapt/GuidingCenter.py:593-606)."""
         from . import flutils

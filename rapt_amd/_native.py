@@ -120,6 +120,7 @@ def lib():
                                 dp, dp]
     L.rapt_field_expr_set.argtypes = [vp, C.c_int32, C.c_int32, i32p, C.c_int32, dp]
     L.rapt_field_expr_mode.argtypes = [vp, C.c_int32]
+    L.rapt_rhs_eval.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int64, dp, dp, dp]
     L.rapt_field_expr_compile.argtypes = [C.c_int32, C.c_int32, i32p, C.c_char_p, C.c_int32,
                                           C.c_char_p, C.c_int64]
     L.rapt_eye.argtypes = [vp, C.c_int64, dp, dp, C.c_double, dp, i32p]
@@ -143,7 +144,7 @@ def lib():
                "rapt_batch_set_rows", "rapt_batch_run", "rapt_batch_download", "rapt_trace",
                "rapt_batch_reset", "rapt_batch_set_cost", "rapt_eye", "rapt_particle_gc",
                "rapt_grid_set", "rapt_field_expr_set", "rapt_trace_device_noalloc",
-               "rapt_field_expr_mode", "rapt_field_expr_compile"):
+               "rapt_field_expr_mode", "rapt_field_expr_compile", "rapt_rhs_eval"):
         getattr(L, nm).restype = C.c_int
     _lib = L
     return L
@@ -222,6 +223,22 @@ class Context:
         same bit for bit."""
         self.check(lib().rapt_field_expr_mode(self.h, EXPR_COMPILED if compiled else
                                               EXPR_INTERPRETED), "rapt_field_expr_mode")
+
+    def rhs_eval(self, desc, mode, ty, k, eom=0, enforce_equatorial=False):
+        """The tracers' right-hand sides at the rows ty ([n][7] Particle,
+        [n][5] GuidingCenter) with per-row constants k ([n][3] (mass, charge,
+        gamma m) or [n][4] (mass, charge, mu, v))."""
+        self.set_field(desc)
+        wi, wk, wo = (7, 3, 6) if mode == MODE_PARTICLE else (5, 4, 4)
+        ty = np.ascontiguousarray(np.atleast_2d(ty), dtype=np.float64)
+        n = ty.shape[0]
+        k = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(k), (n, wk)), dtype=np.float64)
+        assert ty.shape == (n, wi)
+        out = np.zeros((n, wo))
+        e = EOM[eom] if isinstance(eom, str) else int(eom)
+        self.check(lib().rapt_rhs_eval(self.h, mode, e, 1 if enforce_equatorial else 0, n,
+                                       _dp(ty), _dp(k), _dp(out)), "rapt_rhs_eval")
+        return out
 
     def field_eval(self, desc, what, tpos):
         self.set_field(desc)

@@ -538,6 +538,31 @@ __global__ __launch_bounds__(256) void k_field_eval(FieldC F, int what, int64_t 
   }
 }
 
+// ---- batched right-hand sides (the tracers' equations of motion) ---------------
+// mode KIND_PARTICLE: ty [n][7] = (t, x, y, z, px, py, pz), k [n][3] = (mass,
+// charge, gamma m frozen for a static field), out [n][6]; KIND_GC: ty [n][5]
+// = (t, x, y, z, ppar), k [n][4] = (mass, charge, mu, v), out [n][4].  The
+// functions the tracer kernels integrate, one evaluation per row.
+template <int M>
+__global__ __launch_bounds__(256) void k_rhs_eval(FieldC F, int mode, int eom, int eq, int64_t n,
+                                                  const double* __restrict__ ty,
+                                                  const double* __restrict__ k,
+                                                  double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (mode == KIND_PARTICLE) {
+    const double* r = ty + 7 * i;
+    double gm = k[3 * i + 2];
+    rhs_lorentz<M>(F, k[3 * i + 1], k[3 * i], gm, eq != 0, r[0], r + 1, out + 6 * i);
+  } else {
+    const double* r = ty + 5 * i;
+    const double mass = k[4 * i], charge = k[4 * i + 1], mu = k[4 * i + 2], v = k[4 * i + 3];
+    const GcK K = gc_consts(eom, charge, mass, v, mu);
+    rhs_gc<M>(F, eom, F.is_static != 0, charge, mass, mu, v, K, eq != 0, r[0], r + 1,
+              out + 4 * i);
+  }
+}
+
 // the one-lane GC kernel's TCB + static-field instantiation applies
 // (VarEarthDipole is never static)
 template <int M>

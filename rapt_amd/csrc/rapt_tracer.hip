@@ -123,6 +123,14 @@ static void launch_field_eval(rapt_ctx* c, int what, int64_t n, const double* dt
                      n, dt, dout, w);
 }
 
+template <int M>
+static void launch_rhs_eval(rapt_ctx* c, int mode, int eom, int eq, int64_t n, const double* ty,
+                            const double* k, double* out) {
+  const int B = 256;
+  hipLaunchKernelGGL(k_rhs_eval<M>, dim3((unsigned)((n + B - 1) / B)), dim3(B), 0, c->stream,
+                     c->field, mode, eom, eq, n, ty, k, out);
+}
+
 static CtlParams ctl(double safety, double dfactor, double ifactor, double beta, bool dop853) {
   CtlParams p;
   p.safe = safety == 0.0 ? 0.9 : safety;
@@ -445,6 +453,53 @@ int rapt_grid_set(rapt_ctx* c, int32_t nx, int32_t ny, int32_t nz, int32_t nt, c
   c->grid_set = true;
   if (c->field_set && c->field.model == RAPT_FIELD_GRID) c->field.grid = G;
   return RAPT_OK;
+}
+
+int rapt_rhs_eval(rapt_ctx* c, int32_t mode, int32_t eom, int32_t enforce_equatorial, int64_t n,
+                  const double* ty, const double* k, double* out) {
+  if (!c || n < 0 || (n && (!ty || !k || !out))) return RAPT_E_ARG;
+  if (!c->field_set) return set_err(c, RAPT_E_STATE, "no field set");
+  if (mode != RAPT_MODE_PARTICLE && mode != RAPT_MODE_GC)
+    return set_err(c, RAPT_E_ARG, "rhs: mode must be PARTICLE or GC");
+  if (eom < 0 || eom > 2) return set_err(c, RAPT_E_ARG, "bad eom %d", eom);
+  if (n == 0) return RAPT_OK;
+  const int wi = mode == RAPT_MODE_PARTICLE ? 7 : 5, wk = mode == RAPT_MODE_PARTICLE ? 3 : 4,
+            wo = mode == RAPT_MODE_PARTICLE ? 6 : 4;
+  HIP_OK(c, hipSetDevice(c->device));
+  double* mem = nullptr;
+  if (hipMalloc(&mem, sizeof(double) * (size_t)(wi + wk + wo) * n) != hipSuccess)
+    return set_err(c, RAPT_E_OOM, "rhs: allocation failed");
+  double *dty = mem, *dk = mem + (size_t)wi * n, *dout = mem + (size_t)(wi + wk) * n;
+  int rc = RAPT_OK;
+  if (hipMemcpyAsync(dty, ty, sizeof(double) * wi * n, hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(dk, k, sizeof(double) * wk * n, hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess) {
+    rc = set_err(c, RAPT_E_HIP, "rhs: H2D copy failed");
+  } else {
+    const int kind = mode == RAPT_MODE_PARTICLE ? KIND_PARTICLE : KIND_GC;
+    const int eq = enforce_equatorial ? 1 : 0;
+    switch (c->field.model) {
+      case 0: launch_rhs_eval<0>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 1: launch_rhs_eval<1>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 2: launch_rhs_eval<2>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 3: launch_rhs_eval<3>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 4: launch_rhs_eval<4>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 5: launch_rhs_eval<5>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 6: launch_rhs_eval<6>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 7: launch_rhs_eval<7>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 8: launch_rhs_eval<8>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 9: launch_rhs_eval<9>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 10: launch_rhs_eval<10>(c, kind, eom, eq, n, dty, dk, dout); break;
+    }
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(out, dout, sizeof(double) * wo * n, hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = set_err(c, RAPT_E_HIP, "rhs evaluation failed");
+  }
+  (void)hipFree(mem);
+  return rc;
 }
 
 int rapt_field_eval(rapt_ctx* c, int what, int64_t n, const double* tpos, double* out) {
