@@ -513,3 +513,19 @@ def test_compiled_program_and_constant_switches(user_fields):
     for x, y in zip(runs[0].rows, runs[3].rows):
         np.testing.assert_array_equal(x, y)
     assert not all(np.array_equal(x, y) for x, y in zip(runs[0].rows, runs[1].rows))
+
+
+def test_compiled_equatorial_bounce_center(user_fields):
+    """BounceCenter with isequatorial set in the user-written two-dipole field:
+    the compiled k_bounce_center_eq equals the interpreted one bit for bit and
+    the built-in DoubleDipole's run within 1e-6."""
+    def run(f):
+        bc = rapt.BounceCenter([5 * K.Re, 0.0, 0.0], 1e7, 0.0, 90.0, K.m_pr, K.e, f)
+        bc.isequatorial = True
+        bc.advance(200.0)
+        return bc.trajectory
+    ti, tc = _both_modes(lambda: run(user_fields["TwinDipole"]()))
+    np.testing.assert_array_equal(ti, tc)
+    tb = run(F.DoubleDipole())
+    assert tc.shape == tb.shape and len(tc) >= 3
+    np.testing.assert_allclose(tc[:, 1:], tb[:, 1:], rtol=0, atol=1e-6 * K.Re)
