@@ -214,9 +214,9 @@ int rapt_field_expr_set(rapt_ctx *ctx, int32_t nb, int32_t ne, const int32_t *co
  * code; rapt_trace_device_noalloc uses them only once an earlier run has
  * loaded them (loading allocates), and interprets otherwise.
  * RAPT_EXPR_INTERPRETED: the built-in kernels interpret the program.
- * The field-line entry points (rapt_halfbounce, rapt_eye, rapt_bounce_*)
- * compile their kernels in the same way; field evaluation and diagnostics
- * always interpret. */
+ * The other entry points that take the field (rapt_field_eval,
+ * rapt_rhs_eval, rapt_particle_gc, rapt_halfbounce, rapt_eye,
+ * rapt_bounce_*) compile their kernels in the same way. */
 enum rapt_expr_mode { RAPT_EXPR_INTERPRETED = 0, RAPT_EXPR_COMPILED = 1 };
 int rapt_field_expr_mode(rapt_ctx *ctx, int32_t mode);
 
@@ -226,7 +226,8 @@ int rapt_field_expr_mode(rapt_ctx *ctx, int32_t mode);
  * Particle, GuidingCenter, static Tao-Chan-Brizard GuidingCenter and
  * Adaptive kernels; 4-9 GuidingCenter lane pairs, quads and octets (generic,
  * static TCB); 10-11 the mixed layout; 12-15 half-bounce path / eye, labels,
- * BounceCenter, equatorial BounceCenter.  The message (the compiler's log on
+ * BounceCenter, equatorial BounceCenter; 16-18 field evaluation, right-hand
+ * sides, per-row diagnostics.  The message (the compiler's log on
  * failure) goes to log (log_size bytes, NUL-terminated).  No counterpart in
  * the reference: a check, or a warm-up ahead of the first run. */
 int rapt_field_expr_compile(int32_t nb, int32_t ne, const int32_t *code, const char *arch,

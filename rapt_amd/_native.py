@@ -356,7 +356,7 @@ class Context:
 def expr_compile(prog, arch="gfx950", kernels=0):
     """Compiles kernels of a traced user-field program for arch (no device
     needed; the code is cached for the process).  kernels: the library's bit
-    mask (0: all sixteen; 1: the one-lane Particle kernel; ...).  Returns the
+    mask (0: all nineteen; 1: the one-lane Particle kernel; ...).  Returns the
     library's message; raises RuntimeError with the compiler's log on
     failure."""
     code = np.ascontiguousarray(prog.code, dtype=np.int32)

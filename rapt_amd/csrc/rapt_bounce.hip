@@ -94,7 +94,7 @@ static int run_bounce(rapt_ctx* c, int64_t n, const double* in, int in_w, const 
             BounceArgs a = A;
             LabelArgs lb = lab ? *lab : LabelArgs{};
             void* args[] = {&a, &lb};
-            rc = rapt_jit_run_line(c, lab ? RAPT_JIT_LABELS : RAPT_JIT_HALFBOUNCE,
+            rc = rapt_jit_run_slot(c, lab ? RAPT_JIT_LABELS : RAPT_JIT_HALFBOUNCE,
                                    (unsigned)((A.lanes + 63) / 64), 64, args);
           } else {
             lab ? launch_lab<10>(c, A, *lab) : launch_hb<10>(c, A);
@@ -267,7 +267,7 @@ extern "C" int rapt_bounce_center_advance(rapt_ctx* c, int64_t n, double* tpos, 
             BounceArgs a = A;
             BCArgs cc = C;
             void* args[] = {&a, &cc};
-            rc = rapt_jit_run_line(c, RAPT_JIT_BC, (unsigned)((A.lanes + 63) / 64), 64, args);
+            rc = rapt_jit_run_slot(c, RAPT_JIT_BC, (unsigned)((A.lanes + 63) / 64), 64, args);
           } else {
             launch_bc<10>(c, A, C);
           }
@@ -398,7 +398,7 @@ extern "C" int rapt_bounce_center_equatorial(rapt_ctx* c, int64_t n, double* tpo
             BCArgs cc = C;
             EqArgs e = E;
             void* args[] = {&f, &cc, &e};
-            rc = rapt_jit_run_line(c, RAPT_JIT_BC_EQ, (unsigned)((C.n + 63) / 64), 64, args);
+            rc = rapt_jit_run_slot(c, RAPT_JIT_BC_EQ, (unsigned)((C.n + 63) / 64), 64, args);
           } else {
             launch_bc_eq<10>(c, C, E);
           }

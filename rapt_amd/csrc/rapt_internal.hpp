@@ -12,11 +12,14 @@
 #include "rapt_kargs.hpp"
 
 // kernels of a user-written field compiled at run time (rapt_jit.hip)
-#define RAPT_JIT_SLOTS 16
+#define RAPT_JIT_SLOTS 19
 #define RAPT_JIT_HALFBOUNCE 12  // field-line kernels among them
 #define RAPT_JIT_LABELS 13
 #define RAPT_JIT_BC 14
 #define RAPT_JIT_BC_EQ 15
+#define RAPT_JIT_FIELD_EVAL 16  // field evaluation, right-hand sides, diagnostics
+#define RAPT_JIT_RHS_EVAL 17
+#define RAPT_JIT_PARTICLE_GC 18
 
 // Grid of a lane-group GuidingCenter launch: single layouts (L = 2, 4, 8)
 // `groups` resident groups of L lanes; mixed layouts (lanes_per_particle 3,
@@ -84,7 +87,7 @@ bool rapt_jit_launch(rapt_ctx* c, int kind, bool sp, const rapt::KArgs& A, int b
                      int64_t lanes);
 bool rapt_jit_launch_gcn(rapt_ctx* c, int lanes_per_particle, bool sp, const rapt::KArgs& A,
                          int block, int64_t groups, int64_t quads, int64_t pairs);
-int rapt_jit_run_line(rapt_ctx* c, int slot, unsigned grid, unsigned block, void** args);
+int rapt_jit_run_slot(rapt_ctx* c, int slot, unsigned grid, unsigned block, void** args);
 
 #define HIP_OK(ctx, call)                                                              \
   do {                                                                                 \

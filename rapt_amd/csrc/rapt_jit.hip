@@ -8,8 +8,9 @@
 // same values as the interpreter's case for its opcode, the constants read
 // from the program's table in device memory -- and the one-lane tracer
 // kernels k_trace<F_EXPR, kind, FP64> and the lane-group GuidingCenter
-// kernels k_trace_gcn / k_trace_gcmix, and the field-line kernels (half-
-// bounce path, eye, surrogate labels, BounceCenter) are compiled for it with hipRTC from
+// kernels k_trace_gcn / k_trace_gcmix, the field-line kernels (half-bounce
+// path, eye, surrogate labels, BounceCenter), field evaluation, right-hand
+// sides and the row diagnostics are compiled for it with hipRTC from
 // the library's own headers (embedded when the library is built), so the
 // field is inline code like an analytic model's and the kernels run at two
 // waves per SIMD.  The trajectories are the interpreter's one-lane kernel's,
@@ -44,15 +45,18 @@ namespace {
 // Tao-Chan-Brizard one) and Adaptive (slots 0-3, rapt_trace_kernel.hpp);
 // lane pairs, quads and octets (4-9) and the mixed layout (10-11) for
 // GuidingCenter (rapt_gcn_kernel.hpp); the field-line kernels (12-15,
-// rapt_bounce_kernel.hpp)
+// rapt_bounce_kernel.hpp); field evaluation and right-hand sides (16-17,
+// rapt_trace_kernel.hpp) and the row diagnostics (18, rapt_diag_kernel.hpp)
 static_assert(F_EXPR == 10 && KIND_PARTICLE == 0 && KIND_GC == 1 && KIND_ADAPTIVE == 2 &&
                   RAPT_PREC_FP64 == 0,
               "kernel names below spell these values");
 constexpr int kSlots = RAPT_JIT_SLOTS;
 constexpr int kFirstGroupSlot = 4;
 constexpr int kFirstLineSlot = RAPT_JIT_HALFBOUNCE;
+constexpr int kFirstEvalSlot = RAPT_JIT_FIELD_EVAL;
 static_assert(RAPT_JIT_HALFBOUNCE == 12 && RAPT_JIT_LABELS == 13 && RAPT_JIT_BC == 14 &&
-                  RAPT_JIT_BC_EQ == 15 && kSlots == 16,
+                  RAPT_JIT_BC_EQ == 15 && RAPT_JIT_FIELD_EVAL == 16 && RAPT_JIT_RHS_EVAL == 17 &&
+                  RAPT_JIT_PARTICLE_GC == 18 && kSlots == 19,
               "slot numbers below");
 const char* const kKernelNames[kSlots] = {
     "rapt::k_trace<10, 0, 0, false>", "rapt::k_trace<10, 1, 0, false>",
@@ -62,7 +66,9 @@ const char* const kKernelNames[kSlots] = {
     "rapt::k_trace_gcn<10, 8, false>", "rapt::k_trace_gcn<10, 8, true>",
     "rapt::k_trace_gcmix<10, false>",  "rapt::k_trace_gcmix<10, true>",
     "rapt::k_halfbounce<10>",          "rapt::k_bounce_labels<10>",
-    "rapt::k_bounce_center<10>",       "rapt::k_bounce_center_eq<10>"};
+    "rapt::k_bounce_center<10>",       "rapt::k_bounce_center_eq<10>",
+    "rapt::k_field_eval<10>",          "rapt::k_rhs_eval<10>",
+    "rapt::k_particle_gc<10>"};
 
 int jit_slot(int kind, bool sp) {
   return kind == KIND_PARTICLE ? 0 : (kind == KIND_GC ? (sp ? 2 : 1) : 3);
@@ -155,7 +161,9 @@ bool make_source(int32_t nb, int32_t ne, const int32_t* code, int slot, std::str
   src += "}  // namespace rapt\n";
   src += slot < kFirstGroupSlot ? "#include \"rapt_trace_kernel.hpp\"\n"
          : slot < kFirstLineSlot ? "#include \"rapt_gcn_kernel.hpp\"\n"
-                                 : "#include \"rapt_bounce_kernel.hpp\"\n";
+         : slot < kFirstEvalSlot ? "#include \"rapt_bounce_kernel.hpp\"\n"
+         : slot == RAPT_JIT_PARTICLE_GC ? "#include \"rapt_diag_kernel.hpp\"\n"
+                                        : "#include \"rapt_trace_kernel.hpp\"\n";
   return true;
 }
 
@@ -322,10 +330,11 @@ bool rapt_jit_launch_gcn(rapt_ctx* c, int lanes_per_particle, bool sp, const KAr
   return true;
 }
 
-// A field-line kernel of the context's program (slot RAPT_JIT_HALFBOUNCE ..
-// RAPT_JIT_BC_EQ): compiled and loaded when first needed, then launched with
-// the built-in launch's grid and arguments.
-int rapt_jit_run_line(rapt_ctx* c, int slot, unsigned grid, unsigned block, void** args) {
+// A kernel of the context's program outside the tracer launches (the
+// field-line kernels, field evaluation, right-hand sides, row diagnostics:
+// slots RAPT_JIT_HALFBOUNCE onward): compiled and loaded when first needed,
+// then launched with the built-in launch's grid and arguments.
+int rapt_jit_run_slot(rapt_ctx* c, int slot, unsigned grid, unsigned block, void** args) {
   const int rc = prepare_slot(c, slot);
   if (rc) return rc;
   HIP_OK(c, hipModuleLaunchKernel(c->jit_fn[slot], grid, 1, 1, block, 1, 1, 0, c->stream, args,

@@ -490,12 +490,24 @@ int rapt_rhs_eval(rapt_ctx* c, int32_t mode, int32_t eom, int32_t enforce_equato
       case 7: launch_rhs_eval<7>(c, kind, eom, eq, n, dty, dk, dout); break;
       case 8: launch_rhs_eval<8>(c, kind, eom, eq, n, dty, dk, dout); break;
       case 9: launch_rhs_eval<9>(c, kind, eom, eq, n, dty, dk, dout); break;
-      case 10: launch_rhs_eval<10>(c, kind, eom, eq, n, dty, dk, dout); break;
+      case 10:
+        if (c->expr_mode == RAPT_EXPR_COMPILED) {  // compiled at run time
+          FieldC f = c->field;
+          int32_t km = kind, e = eom, q = eq;
+          int64_t nn = n;
+          const double *a0 = dty, *a1 = dk;
+          double* a2 = dout;
+          void* args[] = {&f, &km, &e, &q, &nn, &a0, &a1, &a2};
+          rc = rapt_jit_run_slot(c, RAPT_JIT_RHS_EVAL, (unsigned)((n + 255) / 256), 256, args);
+        } else {
+          launch_rhs_eval<10>(c, kind, eom, eq, n, dty, dk, dout);
+        }
+        break;
     }
-    if (hipGetLastError() != hipSuccess ||
+    if (rc == RAPT_OK && (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(out, dout, sizeof(double) * wo * n, hipMemcpyDeviceToHost, c->stream) !=
             hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
+        hipStreamSynchronize(c->stream) != hipSuccess))
       rc = set_err(c, RAPT_E_HIP, "rhs evaluation failed");
   }
   (void)hipFree(mem);
@@ -531,13 +543,27 @@ int rapt_field_eval(rapt_ctx* c, int what, int64_t n, const double* tpos, double
       case 7: launch_field_eval<7>(c, what, n, dt, dout, w); break;
       case 8: launch_field_eval<8>(c, what, n, dt, dout, w); break;
       case 9: launch_field_eval<9>(c, what, n, dt, dout, w); break;
-      case 10: launch_field_eval<10>(c, what, n, dt, dout, w); break;
+      case 10:
+        if (c->expr_mode == RAPT_EXPR_COMPILED) {  // compiled at run time
+          FieldC f = c->field;
+          int wh = what, ww = w;
+          int64_t nn = n;
+          const double* a0 = dt;
+          double* a1 = dout;
+          void* args[] = {&f, &wh, &nn, &a0, &a1, &ww};
+          rc = rapt_jit_run_slot(c, RAPT_JIT_FIELD_EVAL, (unsigned)((n + 255) / 256), 256, args);
+        } else {
+          launch_field_eval<10>(c, what, n, dt, dout, w);
+        }
+        break;
     }
-    if (hipGetLastError() != hipSuccess) {
+    // (rc set: compiling or loading the user field's kernel failed)
+    if (rc == RAPT_OK && hipGetLastError() != hipSuccess) {
       rc = set_err(c, RAPT_E_HIP, "field kernel launch failed");
-    } else if (hipMemcpyAsync(out, dout, sizeof(double) * w * n, hipMemcpyDeviceToHost,
-                              c->stream) != hipSuccess ||
-               hipStreamSynchronize(c->stream) != hipSuccess) {
+    } else if (rc == RAPT_OK &&
+               (hipMemcpyAsync(out, dout, sizeof(double) * w * n, hipMemcpyDeviceToHost,
+                               c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess)) {
       rc = set_err(c, RAPT_E_HIP, "field eval failed");
     }
   }

@@ -263,7 +263,7 @@ def test_user_field_program_compiles_for_gfx950(user_fields):
     from rapt_amd import _native as N
     prog = user_fields["ChargedDipole"]().descriptor()["expr"]
     # the one-lane Particle kernel and the half-bounce kernel (the others
-    # build from the same headers; all sixteen take a few minutes here)
+    # build from the same headers; all of them take a few minutes here)
     msg = N.expr_compile(prog, kernels=(1 << 0) | (1 << 12))
     assert "2 kernels" in msg, msg
     code = prog.code.copy()
