@@ -561,3 +561,46 @@ def test_compiled_field_eval_rhs_diagnostics(user_fields, name):
     a, b = _both_modes(lambda: ctx.particle_gc(d, ty, K.m_pr, K.e))
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_compiled_every_opcode():
+    """A field that uses every operation the tracer records (the library
+    functions included): compiled values equal the interpreted ones bit for
+    bit, and the host evaluation of the same program to a few ulp."""
+    from rapt_amd import fieldexpr as FE
+
+    class AllOps(rapt.fields._Field):
+        def __init__(self):
+            rapt.fields._Field.__init__(self)
+            self.static = False
+
+        def B(self, tpos):
+            t, x, y, z = tpos
+            u = x / 1e7
+            w = y / 1e7
+            s = z / 1e7
+            a = np.tan(0.3 * u) + np.sinh(0.2 * w) + np.cosh(0.1 * s) + np.arctan(u)
+            b = np.arcsin(0.5 * np.tanh(w)) + np.arccos(0.4 * np.tanh(s)) + np.arctan2(w, u)
+            c = np.log10(1.0 + u * u) + np.log(2.0 + w * w) + np.hypot(u, s) + abs(s) ** 1.5
+            d = np.maximum(u, w) - np.minimum(w, s) + np.sign(s) + np.clip(u, -0.5, 0.5)
+            return 1e-9 * np.array([a + np.exp(-t / 10.0), b * np.sin(t) + c, d * np.cos(u) +
+                                    np.sqrt(1.0 + s * s) - (-u)])
+
+        def E(self, tpos):
+            t, x, y, z = tpos
+            return 1e-4 * np.array([np.sin(x / 1e7), 0.0 * y, np.cos(z / 1e7) ** 2])
+    f = AllOps()
+    d = f.descriptor()
+    ops = set(int(o) for o in d["expr"].code[:, 0])
+    assert ops == set(range(27)), sorted(set(range(27)) - ops)
+    rng = np.random.default_rng(43)
+    tp = np.c_[rng.uniform(0, 5, 64), rng.uniform(-2, 2, (64, 3)) * 1e7]
+    tp[:2, 3] = [0.0, -0.0]
+    ctx = N.context()
+    for w in ("B", "E", "gradB", "curlb", "dbdt"):
+        a, b = _both_modes(lambda: ctx.field_eval(d, w, tp))
+        np.testing.assert_array_equal(a, b, err_msg=w)
+    got = ctx.field_eval(d, "B", tp)
+    for i in range(len(tp)):
+        want = FE.evaluate(d["expr"], tp[i])[0]
+        np.testing.assert_allclose(got[i], want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
