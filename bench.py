@@ -384,7 +384,7 @@ def run_grid(args):
 def run_user_field(args):
     """The reference's plugin contract at speed: EarthDipole + (0, 0, -10 nT)
     written as a user _Field subclass (B overridden in Python, calling the
-    parent's B), config 1's protons at n = 16,384, Particle.advance(1).  The
+    parent's B), config 1's protons at n = 65,536, Particle.advance(1).  The
     program compiled into the tracer kernel at run time against the same
     program interpreted (bit-identical rows) and the built-in model of the
     same field (same formula, different rounding)."""
@@ -399,7 +399,7 @@ def run_user_field(args):
         def B(self, tpos):
             return rapt.fields.EarthDipole.B(self, tpos) + self._Bu
 
-    n = args.n or 16384
+    n = args.n or 65536
     wl = W.dipole_protons_1mev(n=n)
     g = 1 / np.sqrt(1 - np.sum(wl["vel"] ** 2, axis=1) / rapt.c ** 2)
     last7 = np.zeros((n, 7))
