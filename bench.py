@@ -774,7 +774,8 @@ def run_surrogate(args):
     """Learned bounce-averaged drift surrogate in EarthDipole + (0,0,-10 nT):
     protons; queries (L U[2,8], alpha_eq U[10,89] deg, log10 K[eV] U[4,7]).
     Training set 2^20 (seed 4), held-out 2^16 (seed 104), inference 2^24
-    (seed 204).  Labels and fallback: traced bounce quantities on the GPU.
+    (seed 204), and a second 2^16 check set (seed 105) that no setting was
+    chosen on.  Labels and fallback: traced bounce quantities on the GPU.
     value = inference queries/s including the traced fallback of flagged
     queries; compared with the traced rate."""
     import rapt_amd as rapt
@@ -789,7 +790,7 @@ def run_surrogate(args):
         # the inference kernels alone (their work does not depend on the
         # weights): an untrained ensemble, 2^24 queries, for profiling
         qin = qs(1 << 24, 204)
-        s = SG.Surrogate(f, members=4, seed=0)
+        s = SG.Surrogate(f, members=args.members, seed=0)
         s.infer(qin[:1 << 16])
         kms = []
         for i in range(max(args.steps, 1)):
@@ -800,8 +801,8 @@ def run_surrogate(args):
         res = base_line("surrogate inference kernel time, untrained ensemble (profiling)",
                         len(qin) / (ms / 1e3), "queries/s", args, ms / 1e3 * len(kms),
                         "bf16 MFMA / fp32 accumulate",
-                        {"workload": "MLP ensemble 4 x 3-256-256-256-3 SiLU, 2^24 queries, "
-                                     "untrained weights", "queries": len(qin)})
+                        {"workload": "MLP ensemble %d x 3-256-256-256-3 SiLU, 2^24 queries, "
+                                     "untrained weights" % args.members, "queries": len(qin)})
         res["ms_per_step"] = ms
         res["inference_kernel_ms"] = kms
         return res
@@ -814,7 +815,7 @@ def run_surrogate(args):
     log("traced labels: %d in %.2f s (%.3g queries/s), ok %.4f" % (ntr, t_lab, traced_rate,
                                                                     np.mean(st == 1)))
     truth, st_te = SG.traced_labels(f, qte)
-    s = SG.Surrogate(f, members=4, seed=0)
+    s = SG.Surrogate(f, members=args.members, seed=0)
     t0 = time.perf_counter()
     hist = s.fit(qtr[st == 1], lab[st == 1], epochs=args.epochs, batch=4096, log=log)
     t_fit = time.perf_counter() - t0
@@ -851,17 +852,18 @@ def run_surrogate(args):
     # inputs resident in HBM: the inference kernel time, plus the traced
     # fallback of the flagged queries at the traced rate
     value = len(qin) / (kms / 1e3 + idx.size / traced_rate)
-    flop = 4 * 265216.0 * len(qin)
+    flop = args.members * 265216.0 * len(qin)
     per_out = lambda e: {"median": [float(x) for x in np.median(e, axis=0)],
                          "p99": [float(x) for x in np.percentile(e, 99, axis=0)],
                          "max": [float(x) for x in np.max(e, axis=0)],
                          "frac_all_le_1e-3": float(np.mean(np.all(e <= 1e-3, axis=1)))}
     res = base_line("surrogate bounce-drift evals/s at <=1e-3 err (1 GPU)", value, "queries/s",
                     args, t_inf + t_fb, "bf16 MFMA / fp32 accumulate",
-                    {"workload": "EarthDipole + (0,0,-10 nT), protons; MLP ensemble 4 x "
+                    {"workload": "EarthDipole + (0,0,-10 nT), protons; MLP ensemble %d x "
                                  "3-256-256-256-3 SiLU; train 2^20 traced samples, held-out "
                                  "2^16, inference 2^24 queries, fallback when a member deviates "
-                                 "> 1e-3 from the ensemble mean", "queries": len(qin)})
+                                 "> 1e-3 from the ensemble mean" % args.members,
+                     "queries": len(qin)})
     res["steps"], res["warmup"] = 1, 1
     res["ms_per_step"] = (kms / 1e3 + idx.size / traced_rate) * 1e3
     res["traced_queries_per_s"] = traced_rate
@@ -882,7 +884,15 @@ def run_surrogate(args):
     res["roofline"] = {"bound": "mfma", "achieved": flop / (kms / 1e3) / 1e12,
                        "peak": 2500.0, "unit": "TFLOP/s",
                        "frac": flop / (kms / 1e3) / 2.5e15, "traffic": None,
-                       "flop_per_query": 4 * 265216}
+                       "flop_per_query": args.members * 265216}
+    # an independent check set (seed 105) that chose nothing: neither the
+    # threshold nor the member count and epochs
+    qva = qs(1 << 16, 105)
+    truth_va, st_va = SG.traced_labels(f, qva)
+    p_va, fl_va, _ = s.predict(qva, thresh)
+    e_va = np.abs(p_va - truth_va) / np.abs(truth_va)
+    res["check_set_with_fallback_rel_err"] = per_out(e_va[st_va == 1])
+    res["check_set_flagged_frac"] = float(np.mean(fl_va))
     s.close()
     return res
 
@@ -893,7 +903,9 @@ def main():
     ap.add_argument("--workload", default="gc_doubledipole",
                     choices=("gc_doubledipole", "particle_dipole", "adaptive_sheet",
                              "mixed_precision", "surrogate", "grid", "user_field"))
-    ap.add_argument("--epochs", type=int, default=100, help="surrogate training epochs")
+    ap.add_argument("--epochs", type=int, default=200, help="surrogate training epochs")
+    ap.add_argument("--members", type=int, default=2, choices=(2, 3, 4),
+                    help="surrogate: ensemble members")
     ap.add_argument("--dump", default="", help="surrogate: write held-out results to this .npz")
     ap.add_argument("--infer-only", action="store_true",
                     help="surrogate: time only the inference kernels (untrained ensemble)")
@@ -980,7 +992,8 @@ def other_configs(args):
             ("surrogate", run_surrogate,
              ("value", "unit", "speedup_vs_traced", "traced_queries_per_s", "inference_kernel_ms",
               "fallback_queries", "fallback_ms", "heldout_surrogate_rel_err",
-              "heldout_with_fallback_rel_err", "heldout_flagged_frac", "roofline", "train_s",
+              "heldout_with_fallback_rel_err", "heldout_flagged_frac", "check_set_with_fallback_rel_err",
+              "check_set_flagged_frac", "roofline", "train_s",
               "config"))):
         a = copy.copy(args)
         a.n, a.lanes, a.block = 0, 0, 0
